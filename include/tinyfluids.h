@@ -1,0 +1,197 @@
+/*
+ * tinyfluids.h -- C ABI of libtfhip.so, the gfx950 (MI355X) implementation of
+ * the jax_tinyfluids time step: first-order Godunov, HLL fluxes, periodic
+ * neighbours, dimensionally split X -> Y -> Z sweeps and the CFL time step.
+ *
+ * Conventions shared by every entry point:
+ *  - State buffers are device pointers to float32 arrays laid out as the
+ *    reference's STATE_TYPE, state[5][nx][ny][nz] in C order (z fastest),
+ *    holding PRIMITIVE variables rho, vx, vy, vz, p at indices 0..4
+ *    (reference tinyfluids/jax_tinyfluids/fluid.py:10-22). Conversions take
+ *    and return the conserved layout rho, mx, my, mz, E.
+ *  - `stream` is a hipStream_t passed as void*; NULL is the default stream.
+ *    Work is enqueued asynchronously unless an entry point says otherwise.
+ *  - `workspace` is device memory of at least tf_workspace_bytes(...) bytes,
+ *    owned by the caller. The library never allocates or frees device memory.
+ *  - Return value: TF_OK (0) or a negative TF_ERR_* code; tf_last_error()
+ *    gives a message (thread-local).
+ *  - Arithmetic: float32 per cell; the signal-speed maximum is exact (a max of
+ *    float32 values), dt = C_CFL * dx / max, t and dt are float64.
+ *
+ * Reference paths below are relative to the reference repository root.
+ */
+#ifndef TINYFLUIDS_H
+#define TINYFLUIDS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TF_OK 0
+#define TF_ERR_ARG (-1)         /* bad argument: null pointer, bad size, bad axis */
+#define TF_ERR_HIP (-2)         /* a HIP runtime call failed */
+#define TF_ERR_NONFINITE (-3)   /* non-finite signal speed seen (negative p, NaN) */
+#define TF_ERR_UNSUPPORTED (-4) /* a shape or option this build does not run */
+
+#define TF_ABI_VERSION 1
+
+/* Library ABI version (TF_ABI_VERSION). No reference counterpart. */
+int tf_abi_version(void);
+
+/* Message of the last error on this thread ("" if none). No reference counterpart. */
+const char *tf_last_error(void);
+
+/* Bytes of workspace the entry points below need for an nx*ny*nz grid.
+ * No reference counterpart (JAX allocates internally). */
+size_t tf_workspace_bytes(int nx, int ny, int nz);
+
+/* Replaces conserved_state_from_primitive (fluid.py:28-67):
+ * cons = (rho, rho*v, p/(gamma-1) + rho*|v|^2/2) for ncells cells.
+ * prim and cons are [5][ncells]; they may alias. */
+int tf_conserved_state_from_primitive(const float *prim, float *cons, int64_t ncells, double gamma, void *stream);
+
+/* Replaces primitive_state_from_conserved (fluid.py:69-103):
+ * prim = (rho, m/rho, (gamma-1)*(E - |m|^2/(2 rho))). May alias. */
+int tf_primitive_state_from_conserved(const float *cons, float *prim, int64_t ncells, double gamma, void *stream);
+
+/* Replaces _cfl_time_step (fluid.py:425-504): writes the float64
+ * dt = C_CFL * grid_spacing / max_signal_speed to the device pointer dt_out,
+ * where max_signal_speed is the maximum over periodic interfaces and axes of
+ * max(|S_R|, |S_L|) with the Davis wave speeds of fluid.py:119-160; it equals
+ * the per-cell maximum of |v_d| + c exactly, which is what is evaluated.
+ * max_speed_out (device, may be NULL) receives that float maximum. */
+int tf_cfl_time_step(const float *prim, int nx, int ny, int nz, double grid_spacing, double gamma, double c_cfl,
+                     double *dt_out, float *max_speed_out, void *workspace, void *stream);
+
+/* Replaces _open_boundaries(state, 1) (fluid.py:382-413): out = in with the
+ * first and last cell along every axis copied from their inner neighbours,
+ * i.e. out[v][x][y][z] = in[v][g(x)][g(y)][g(z)], g = clamp(., 1, n-2).
+ * Requires every n >= 2. in and out must not alias. */
+int tf_open_boundaries(const float *in, float *out, int nx, int ny, int nz, void *stream);
+
+/* Replaces _evolve_state_along_axis (fluid.py:506-557): one split sweep along
+ * axis (1 = x, 2 = y, 3 = z) with the HLL flux at every periodic interface:
+ * U += dt/dx * (F[i-1/2] - F[i+1/2]), then back to primitives. dt is a
+ * device pointer to a float64. in and out must not alias. */
+int tf_evolve_state_along_axis(const float *in, float *out, int nx, int ny, int nz, double grid_spacing,
+                               const double *dt, double gamma, int axis, void *stream);
+
+/* Replaces _evolve_state (fluid.py:560-578): [open boundaries if
+ * handle_boundaries] then the X, Y and Z sweeps, fused into one pass over
+ * HBM. dt is a device pointer to a float64. in and out must not alias.
+ * If max_speed_out is not NULL, the maximum per-cell |v_d| + c of the new
+ * state is max-accumulated into it (as float bits; clear it first). */
+int tf_evolve_state(const float *in, float *out, int nx, int ny, int nz, double grid_spacing, const double *dt,
+                    double gamma, int handle_boundaries, unsigned int *max_speed_out, void *workspace, void *stream);
+
+/* Replaces time_integration(..., shard_mapped=False)
+ * (time_integration.py:94-128, loop at :63-92): while t < t_final (and fewer
+ * than max_steps steps when max_steps >= 0): dt = _cfl_time_step(state);
+ * state = _evolve_state(state, dt); t += dt; n += 1. No dt clipping; a NaN dt
+ * ends the loop after the step that used it, as in the reference.
+ * state and scratch are two device buffers of the state's size; the result
+ * ends in state if *result_in_scratch == 0, else in scratch. Blocks until the
+ * run is done (the reference returns after block_until_ready()).
+ * num_iterations, t_out, result_in_scratch, nonfinite are host pointers
+ * (nonfinite may be NULL). */
+int tf_time_integration(float *state, float *scratch, int nx, int ny, int nz, double grid_spacing, double t_final,
+                        double gamma, int handle_boundaries, int64_t max_steps, double c_cfl, void *workspace,
+                        int64_t *num_iterations, double *t_out, int *result_in_scratch, int *nonfinite,
+                        void *stream);
+
+/* Totals of mass, momentum x/y/z and energy of a primitive state, each cell
+ * converted and summed in float64 in a fixed order (deterministic). out5 is a
+ * device pointer to 5 doubles. No reference counterpart (diagnostic). */
+int tf_conserved_totals(const float *prim, int nx, int ny, int nz, double gamma, double *out5, void *workspace,
+                        void *stream);
+
+/* ------------------------------------------------------------------------
+ * Batched engine: many independent simulations of uneven sizes advanced
+ * together. No reference counterpart (the reference runs one grid per call).
+ * ------------------------------------------------------------------------ */
+
+#define TF_CTRL_BYTES 128
+
+/* One simulation. buf0/buf1 are its two state buffers ([5][nx][ny][nz]
+ * float32 each); after tf_sim_reset the state is in buf0, and after n steps
+ * in buf(n % 2). ctrl is TF_CTRL_BYTES of device memory owned by the caller
+ * that holds the simulation's t, dt, step count and flags on the device. */
+typedef struct tf_sim_desc {
+    float *buf0;
+    float *buf1;
+    void *ctrl;
+    int nx, ny, nz;
+    int handle_boundaries;
+    double grid_spacing;
+    double gamma;
+    double c_cfl;
+    double t_final;     /* +inf for a fixed step count */
+    int64_t max_steps;  /* < 0: no limit */
+} tf_sim_desc;
+
+/* Host view of a simulation's control block (its first 48 bytes). */
+typedef struct tf_sim_status {
+    double t;          /* simulated time reached */
+    double dt;         /* dt of the next step */
+    int64_t n_iter;    /* steps taken; the state is in buf(n_iter % 2) */
+    int32_t active;    /* 1 while the simulation still steps */
+    int32_t nonfinite; /* 1 if a non-finite signal speed was seen */
+    float max_speed;   /* max |v_d| + c of the current state */
+    int32_t pad[3];
+} tf_sim_status;
+
+/* Sets t = 0, n = 0, active, and the first dt from the state in buf0
+ * (the loop head of time_integration.py:63-72). Asynchronous. */
+int tf_sim_reset(const tf_sim_desc *sim, void *stream);
+
+/* Opaque batch plan: the device tables of a set of simulations. */
+typedef struct tf_batch tf_batch;
+
+/* Workspace bytes for a plan over nsims simulations. */
+size_t tf_batch_workspace_bytes(int nsims);
+
+/* Builds a plan; its device tables live in the caller's workspace.
+ * tile_x (x-cells per wave: 2, 4 or 8) and tile_y (rows per y-chunk) choose
+ * the stencil tiling; 0 picks the library default. */
+int tf_batch_create(tf_batch **out, const tf_sim_desc *sims, int nsims, void *workspace, size_t workspace_bytes,
+                    int tile_x, int tile_y, void *stream);
+int tf_batch_destroy(tf_batch *b);
+
+/* Enqueues `steps` batched steps: each is one fused multi-simulation stencil
+ * launch over every simulation's tiles plus one per-simulation dt update.
+ * Simulations that are no longer active are skipped on the device. */
+int tf_batch_step(tf_batch *b, int steps, void *stream);
+
+/* As tf_batch_step, but the `steps` steps are captured once into a HIP graph
+ * (re-captured when `steps` changes) and replayed: one graph launch. */
+int tf_batch_step_graph(tf_batch *b, int steps, void *stream);
+
+/* As tf_batch_step, also recording HIP events around every stencil launch
+ * on `stream`; waits for the steps and returns in *stencil_ms the summed
+ * duration of the stencil launches alone (without the dt updates). */
+int tf_batch_step_timed(tf_batch *b, int steps, float *stencil_ms, void *stream);
+
+/* Copies every simulation's status to host memory (synchronous). */
+int tf_batch_status(tf_batch *b, tf_sim_status *host_out, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Tiling and autotuning of the fused stencil. No reference counterpart.
+ * ------------------------------------------------------------------------ */
+
+/* Tiling used for an nx*ny*nz grid by tf_evolve_state / tf_time_integration. */
+int tf_get_tiling(int nx, int ny, int nz, int *tile_x, int *tile_y);
+int tf_set_tiling(int nx, int ny, int nz, int tile_x, int tile_y);
+
+/* Times the candidate tilings of the fused step on (in -> out), keeps the
+ * fastest for this shape (tf_set_tiling) and reports it. Synchronous. */
+int tf_autotune_evolve(const float *in, float *out, int nx, int ny, int nz, double grid_spacing, double gamma,
+                       void *workspace, int *tile_x, int *tile_y, float *ms_per_step, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TINYFLUIDS_H */

@@ -1,0 +1,154 @@
+"""GPU parity of the HIP path against the float64 oracle (through the C ABI).
+
+Tolerances: float32 device arithmetic against the float64 oracle, measured as
+relative L1 per conserved field, sum|q32 - q64| / sum|q64| (absolute sum for
+fields that are zero in the oracle):
+  - one step / one sweep / conversions: 2e-6
+  - full runs (100-200 steps through shocks): 1e-5
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import c_oracle, np_tinyfluids as npo
+from tinyfluids_amd import ics
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device("cuda:0")
+ONE_STEP_TOL = 2e-6
+RUN_TOL = 1e-5
+
+
+def tf():
+    import tinyfluids_amd
+    return tinyfluids_amd
+
+
+def random_state(shape, seed, amp=0.5):
+    rng = np.random.default_rng(seed)
+    s = np.empty((5,) + tuple(shape))
+    s[0] = 1.0 + amp * rng.uniform(0, 1, shape)
+    s[1:4] = amp * rng.uniform(-1, 1, (3,) + tuple(shape))
+    s[4] = 1.0 + amp * rng.uniform(0, 1, shape)
+    return s.astype(np.float32)
+
+
+def to_dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
+
+
+def rel_l1(gpu_tensor, ref64, gamma):
+    return npo.relative_l1(gpu_tensor.cpu().numpy(), ref64, gamma)
+
+
+def test_library_is_native():
+    import tinyfluids_amd._lib as L
+    lib = L.load()
+    assert lib.tf_abi_version() == 1
+    assert torch.cuda.is_available()
+
+
+@pytest.mark.parametrize("gamma", [5 / 3, 1.4])
+def test_conversions(gamma):
+    s = random_state((6, 7, 9), 1)
+    cons = tf().conserved_state_from_primitive(to_dev(s), gamma)
+    ref = npo.conserved_state_from_primitive(s.astype(np.float64), gamma)
+    np.testing.assert_allclose(cons.cpu().numpy(), ref, rtol=2e-6, atol=1e-7)
+    back = tf().primitive_state_from_conserved(cons, gamma)
+    np.testing.assert_allclose(back.cpu().numpy(), s, rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("shape", [(8, 8, 8), (5, 9, 70), (33, 3, 130)])
+def test_cfl_time_step(shape):
+    s = random_state(shape, 2)
+    dt = tf()._cfl_time_step(to_dev(s), 0.01, 5 / 3)
+    ref = npo._cfl_time_step(s.astype(np.float64), 0.01, 5 / 3)
+    assert abs(float(dt) - ref) <= 1e-6 * ref
+
+
+@pytest.mark.parametrize("axis", [1, 2, 3])
+def test_sweep_along_axis(axis):
+    s = random_state((9, 10, 11), 3)
+    out = tf()._evolve_state_along_axis(to_dev(s), 0.05, 0.004, 5 / 3, axis)
+    ref = npo._evolve_state_along_axis(s.astype(np.float64), 0.05, 0.004, 5 / 3, axis)
+    assert np.all(rel_l1(out, ref, 5 / 3) < ONE_STEP_TOL)
+
+
+@pytest.mark.parametrize("shape", [(4, 4, 4), (7, 9, 70), (16, 12, 62), (3, 17, 125), (9, 2, 5), (1, 6, 8)])
+@pytest.mark.parametrize("hb", [False, True])
+def test_fused_step(shape, hb):
+    if hb and min(shape) < 2:
+        pytest.skip("open boundaries need every dimension >= 2")
+    s = random_state(shape, 4)
+    print(f"fused step shape={shape} handle_boundaries={hb}", flush=True)
+    out = tf()._evolve_state(to_dev(s), 0.05, 0.004, 5 / 3, hb)
+    ref = npo._evolve_state(s.astype(np.float64), 0.05, 0.004, 5 / 3, hb)
+    assert np.all(rel_l1(out, ref, 5 / 3) < ONE_STEP_TOL)
+
+
+@pytest.mark.parametrize("tx", [2, 4, 8])
+def test_fused_tilings_agree_bitwise(tx):
+    """Per-cell arithmetic does not depend on the tiling."""
+    import tinyfluids_amd._lib as L
+    s = to_dev(random_state((12, 40, 130), 5))
+    lib = L.load()
+    base = tf()._evolve_state(s, 0.05, 0.003, 1.4)
+    for ly in (1, 7, 40):
+        L.check(lib.tf_set_tiling(12, 40, 130, tx, ly))
+        out = tf()._evolve_state(s, 0.05, 0.003, 1.4)
+        assert torch.equal(out, base), (tx, ly)
+    L.check(lib.tf_set_tiling(12, 40, 130, 4, 40))
+
+
+def test_fused_matches_split_path():
+    s = to_dev(random_state((10, 11, 12), 6))
+    a = tf()._evolve_state(s, 0.05, 0.004, 5 / 3, True)
+    b = tf()._evolve_state_split(s, 0.05, 0.004, 5 / 3, True)
+    assert torch.allclose(a, b, rtol=2e-6, atol=1e-7)
+
+
+def test_open_boundaries():
+    s = random_state((5, 6, 7), 7)
+    out = tf()._open_boundaries(to_dev(s))
+    ref = c_oracle.open_boundaries(s)
+    assert np.array_equal(out.cpu().numpy(), ref)
+
+
+def test_blast_64_100_steps():
+    """64^3 periodic spherical blast, gamma 5/3, CFL 0.4, 100 steps."""
+    s0, dx = ics.blast((64, 64, 64))
+    g = ics.GAMMA_BLAST
+    out, t, nonfinite = tf().run_steps(to_dev(s0), dx, g, 100)
+    ref, t_ref, _ = c_oracle.run_steps(s0.astype(np.float64), dx, g, 100)
+    err = rel_l1(out, ref, g)
+    print("blast64 rel L1", err, "t", t, t_ref, flush=True)
+    assert not nonfinite
+    assert np.all(err < RUN_TOL)
+    assert abs(t - t_ref) < 1e-5 * t_ref
+    tot0 = c_oracle.conserved_totals(s0, g)
+    tot = tf().conserved_totals(out, g).cpu().numpy()
+    # mass and energy: float32 round-off over 100 steps; momenta stay ~0
+    assert abs(tot[0] - tot0[0]) < 1e-6 * tot0[0]
+    assert abs(tot[4] - tot0[4]) < 1e-5 * tot0[4]
+    assert np.all(np.abs(tot[1:4]) < 1e-6 * tot0[0])
+
+
+@pytest.mark.parametrize("hb", [True, False])
+@pytest.mark.parametrize("n,iters", [(32, 24), (64, 51)])
+def test_iteration_counts(n, iters, hb):
+    s, dx = ics.setup_ics(n, n // 16)
+    _, num = tf().time_integration(to_dev(s), dx, 0.2, 5 / 3, hb, False)
+    assert num == iters
+
+
+def test_nan_ends_loop_like_reference():
+    s, dx = ics.blast((16, 16, 16))
+    s[4, 3, 4, 5] = -1.0  # negative pressure: sound speed NaN, dt NaN
+    _, num = tf().time_integration(to_dev(s), dx, 0.2, 5 / 3, False, False)
+    assert num == 1
+    with pytest.raises(FloatingPointError):
+        tf().time_integration(to_dev(s), dx, 0.2, 5 / 3, False, False, check_finite=True)

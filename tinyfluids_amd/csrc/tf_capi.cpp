@@ -1,0 +1,571 @@
+// C ABI of libtfhip.so (declared in include/tinyfluids.h): argument checks,
+// tiling choice, the device-resident time loop and the batched engine.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/tinyfluids.h"
+#include "tf_internal.h"
+
+namespace tf {
+hipError_t launch_step_direct(int tx, const SimDev &S, const float *in, float *out, const double *dt, double dx,
+                              uint32_t *bits, hipStream_t st);
+hipError_t launch_step_sims(int tx, const SimDev *sims, const int *prefix, int nsims, int total_waves,
+                            hipStream_t st);
+hipError_t launch_finalize_sims(const SimDev *sims, int nsims, hipStream_t st);
+hipError_t launch_finalize_one(Ctrl *C, int first, hipStream_t st);
+hipError_t launch_ctrl_reset(Ctrl *C, double dx, double c_cfl, double t_final, int64_t max_steps, hipStream_t st);
+hipError_t launch_fill_u32(uint32_t *p, uint32_t v, int n, hipStream_t st);
+hipError_t launch_speed(const float *prim, int64_t nc, float gamma, uint32_t *bits, hipStream_t st);
+hipError_t launch_dt_from_bits(const uint32_t *bits, double dx, double c_cfl, double *dt_out, float *max_out,
+                               hipStream_t st);
+hipError_t launch_sweep(int axis, const float *in, float *out, int nx, int ny, int nz, const Gas &g, const double *dt,
+                        double dx, hipStream_t st);
+hipError_t launch_open_bc(const float *in, float *out, int nx, int ny, int nz, hipStream_t st);
+hipError_t launch_p2c(const float *in, float *out, int64_t nc, const Gas &g, hipStream_t st);
+hipError_t launch_c2p(const float *in, float *out, int64_t nc, const Gas &g, hipStream_t st);
+hipError_t launch_totals(const float *prim, int64_t nc, double gamma, double *partials, double *out5,
+                         hipStream_t st);
+hipError_t launch_gather_status(const SimDev *sims, int nsims, Ctrl *dst, hipStream_t st);
+}  // namespace tf
+
+using namespace tf;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define TF_HIP(call)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (call);                                                                        \
+        if (e_ != hipSuccess) return fail(TF_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_));         \
+    } while (0)
+
+#define TF_REQUIRE(cond, ...)                                   \
+    do {                                                        \
+        if (!(cond)) return fail(TF_ERR_ARG, __VA_ARGS__);      \
+    } while (0)
+
+inline hipStream_t S_(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+int check_dims(int nx, int ny, int nz) {
+    TF_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1, "grid dimensions must be >= 1 (got %d x %d x %d)", nx, ny, nz);
+    const double cells = (double)nx * ny * nz;
+    TF_REQUIRE(cells * 5.0 < 9.0e18, "grid too large");
+    TF_REQUIRE((double)nx * ny < 2.0e9 && (double)ny * nz < 2.0e9, "grid planes exceed 2^31 cells");
+    return TF_OK;
+}
+
+Gas make_gas(double gamma, float k = 0.0f) {
+    Gas g;
+    g.gamma = (float)gamma;
+    g.gm1 = (float)gamma - 1.0f;  // gamma is float32 first, as in the reference's float32 runs
+    g.inv_gm1 = 1.0f / g.gm1;
+    g.k = k;
+    return g;
+}
+
+// ------------------------------------------------------------- tiling registry
+std::mutex g_tiling_mu;
+std::map<std::tuple<int, int, int>, std::pair<int, int>> g_tiling;
+
+int default_tile_x(int nx) { return nx >= 4 ? 4 : 2; }
+
+// Rows per y-chunk: enough waves to fill the chip (256 CUs, a few waves per
+// SIMD) with several rounds so the last round is short, without letting the
+// two halo rows per chunk cost much.
+int default_tile_y(int nx, int ny, int nz, int tx) {
+    const long ntx = (nx + tx - 1) / tx, ntz = (nz + kZT - 1) / kZT;
+    const long per_chunk = ntx * ntz;
+    const long target = 256L * 4 * 3 * 6;  // ~6 rounds of full occupancy
+    int ly = ny;
+    while (ly > 32 && per_chunk * ((ny + ly - 1) / ly) < target) ly = (ly + 1) / 2;
+    return std::max(1, ly);
+}
+
+void tiling_for(int nx, int ny, int nz, int &tx, int &ly) {
+    {
+        std::lock_guard<std::mutex> lk(g_tiling_mu);
+        auto it = g_tiling.find({nx, ny, nz});
+        if (it != g_tiling.end()) {
+            tx = it->second.first;
+            ly = it->second.second;
+            return;
+        }
+    }
+    tx = default_tile_x(nx);
+    ly = default_tile_y(nx, ny, nz, tx);
+}
+
+SimDev make_simdev(int nx, int ny, int nz, int open_bc, double gamma, int tx, int ly) {
+    SimDev S;
+    std::memset(&S, 0, sizeof S);
+    S.nx = nx;
+    S.ny = ny;
+    S.nz = nz;
+    S.open_bc = open_bc ? 1 : 0;
+    Gas g = make_gas(gamma);
+    S.gamma = g.gamma;
+    S.gm1 = g.gm1;
+    S.inv_gm1 = g.inv_gm1;
+    S.ly = std::max(1, std::min(ly, ny));
+    S.ntx = (nx + tx - 1) / tx;
+    S.ntz = (nz + kZT - 1) / kZT;
+    S.nyc = (ny + S.ly - 1) / S.ly;
+    S.waves = S.ntx * S.ntz * S.nyc;
+    return S;
+}
+
+bool valid_tx(int tx) { return tx == 2 || tx == 4 || tx == 8; }
+
+int check_open_bc(int open_bc, int nx, int ny, int nz) {
+    if (open_bc) TF_REQUIRE(nx >= 2 && ny >= 2 && nz >= 2, "open boundaries need every dimension >= 2");
+    return TF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tf_abi_version(void) { return TF_ABI_VERSION; }
+
+const char *tf_last_error(void) { return g_err.c_str(); }
+
+size_t tf_workspace_bytes(int nx, int ny, int nz) {
+    (void)nx;
+    (void)ny;
+    (void)nz;
+    return kWsHeaderBytes + (size_t)kTotalsBlocks * 5 * sizeof(double);
+}
+
+int tf_conserved_state_from_primitive(const float *prim, float *cons, int64_t ncells, double gamma, void *stream) {
+    TF_REQUIRE(prim && cons, "null state pointer");
+    TF_REQUIRE(ncells >= 0, "ncells must be >= 0");
+    if (ncells == 0) return TF_OK;
+    TF_HIP(launch_p2c(prim, cons, ncells, make_gas(gamma), S_(stream)));
+    return TF_OK;
+}
+
+int tf_primitive_state_from_conserved(const float *cons, float *prim, int64_t ncells, double gamma, void *stream) {
+    TF_REQUIRE(prim && cons, "null state pointer");
+    TF_REQUIRE(ncells >= 0, "ncells must be >= 0");
+    if (ncells == 0) return TF_OK;
+    TF_HIP(launch_c2p(cons, prim, ncells, make_gas(gamma), S_(stream)));
+    return TF_OK;
+}
+
+int tf_cfl_time_step(const float *prim, int nx, int ny, int nz, double grid_spacing, double gamma, double c_cfl,
+                     double *dt_out, float *max_speed_out, void *workspace, void *stream) {
+    if (int e = check_dims(nx, ny, nz)) return e;
+    TF_REQUIRE(prim && workspace, "null pointer");
+    TF_REQUIRE(dt_out || max_speed_out, "nothing to write: dt_out and max_speed_out are both NULL");
+    auto *H = reinterpret_cast<WsHeader *>(workspace);
+    hipStream_t st = S_(stream);
+    TF_HIP(launch_fill_u32(H->bits, 0u, 1, st));
+    TF_HIP(launch_speed(prim, (int64_t)nx * ny * nz, (float)gamma, H->bits, st));
+    TF_HIP(launch_dt_from_bits(H->bits, grid_spacing, c_cfl, dt_out, max_speed_out, st));
+    return TF_OK;
+}
+
+int tf_open_boundaries(const float *in, float *out, int nx, int ny, int nz, void *stream) {
+    if (int e = check_dims(nx, ny, nz)) return e;
+    TF_REQUIRE(in && out, "null state pointer");
+    TF_REQUIRE(in != out, "in and out must not alias");
+    if (int e = check_open_bc(1, nx, ny, nz)) return e;
+    TF_HIP(launch_open_bc(in, out, nx, ny, nz, S_(stream)));
+    return TF_OK;
+}
+
+int tf_evolve_state_along_axis(const float *in, float *out, int nx, int ny, int nz, double grid_spacing,
+                               const double *dt, double gamma, int axis, void *stream) {
+    if (int e = check_dims(nx, ny, nz)) return e;
+    TF_REQUIRE(in && out && dt, "null pointer");
+    TF_REQUIRE(in != out, "in and out must not alias");
+    TF_REQUIRE(axis >= 1 && axis <= 3, "axis must be 1 (x), 2 (y) or 3 (z), got %d", axis);
+    TF_HIP(launch_sweep(axis, in, out, nx, ny, nz, make_gas(gamma), dt, grid_spacing, S_(stream)));
+    return TF_OK;
+}
+
+int tf_evolve_state(const float *in, float *out, int nx, int ny, int nz, double grid_spacing, const double *dt,
+                    double gamma, int handle_boundaries, unsigned int *max_speed_out, void *workspace,
+                    void *stream) {
+    (void)workspace;
+    if (int e = check_dims(nx, ny, nz)) return e;
+    TF_REQUIRE(in && out && dt, "null pointer");
+    TF_REQUIRE(in != out, "in and out must not alias");
+    if (int e = check_open_bc(handle_boundaries, nx, ny, nz)) return e;
+    int tx, ly;
+    tiling_for(nx, ny, nz, tx, ly);
+    const SimDev S = make_simdev(nx, ny, nz, handle_boundaries, gamma, tx, ly);
+    TF_HIP(launch_step_direct(tx, S, in, out, dt, grid_spacing, max_speed_out, S_(stream)));
+    return TF_OK;
+}
+
+int tf_time_integration(float *state, float *scratch, int nx, int ny, int nz, double grid_spacing, double t_final,
+                        double gamma, int handle_boundaries, int64_t max_steps, double c_cfl, void *workspace,
+                        int64_t *num_iterations, double *t_out, int *result_in_scratch, int *nonfinite,
+                        void *stream) {
+    if (int e = check_dims(nx, ny, nz)) return e;
+    TF_REQUIRE(state && scratch && workspace, "null device pointer");
+    TF_REQUIRE(num_iterations && t_out && result_in_scratch, "null host result pointer");
+    TF_REQUIRE(state != scratch, "state and scratch must be different buffers");
+    if (int e = check_open_bc(handle_boundaries, nx, ny, nz)) return e;
+    hipStream_t st = S_(stream);
+    auto *H = reinterpret_cast<WsHeader *>(workspace);
+    int tx, ly;
+    tiling_for(nx, ny, nz, tx, ly);
+    SimDev S = make_simdev(nx, ny, nz, handle_boundaries, gamma, tx, ly);
+    S.buf[0] = state;
+    S.buf[1] = scratch;
+    S.ctrl = &H->ctrl;
+    const int prefix[2] = {0, S.waves};
+    TF_HIP(hipMemcpyAsync(&H->sim, &S, sizeof S, hipMemcpyHostToDevice, st));
+    TF_HIP(hipMemcpyAsync(H->prefix, prefix, sizeof prefix, hipMemcpyHostToDevice, st));
+    TF_HIP(launch_ctrl_reset(&H->ctrl, grid_spacing, c_cfl, t_final, max_steps, st));
+    TF_HIP(launch_speed(state, (int64_t)nx * ny * nz, (float)gamma, &H->ctrl.max_bits[0], st));
+    TF_HIP(launch_finalize_one(&H->ctrl, 1, st));
+    // Steps run in chunks; a step whose control block says inactive returns
+    // at once, so the host checks the loop condition once per chunk.
+    Ctrl host;
+    int chunk = 4;
+    for (;;) {
+        for (int i = 0; i < chunk; ++i) {
+            TF_HIP(launch_step_sims(tx, &H->sim, H->prefix, 1, S.waves, st));
+            TF_HIP(launch_finalize_sims(&H->sim, 1, st));
+        }
+        TF_HIP(hipMemcpyAsync(&host, &H->ctrl, sizeof host, hipMemcpyDeviceToHost, st));
+        TF_HIP(hipStreamSynchronize(st));
+        if (!host.active) break;
+        chunk = std::min(chunk * 2, 64);
+    }
+    *num_iterations = host.n_iter;
+    *t_out = host.t;
+    *result_in_scratch = (int)(host.n_iter & 1);
+    if (nonfinite) *nonfinite = host.nonfinite;
+    return TF_OK;
+}
+
+int tf_conserved_totals(const float *prim, int nx, int ny, int nz, double gamma, double *out5, void *workspace,
+                        void *stream) {
+    if (int e = check_dims(nx, ny, nz)) return e;
+    TF_REQUIRE(prim && out5 && workspace, "null pointer");
+    double *partials = reinterpret_cast<double *>(static_cast<char *>(workspace) + kWsHeaderBytes);
+    TF_HIP(launch_totals(prim, (int64_t)nx * ny * nz, gamma, partials, out5, S_(stream)));
+    return TF_OK;
+}
+
+// ------------------------------------------------------------------ tiling
+int tf_get_tiling(int nx, int ny, int nz, int *tile_x, int *tile_y) {
+    if (int e = check_dims(nx, ny, nz)) return e;
+    TF_REQUIRE(tile_x && tile_y, "null pointer");
+    tiling_for(nx, ny, nz, *tile_x, *tile_y);
+    return TF_OK;
+}
+
+int tf_set_tiling(int nx, int ny, int nz, int tile_x, int tile_y) {
+    if (int e = check_dims(nx, ny, nz)) return e;
+    TF_REQUIRE(valid_tx(tile_x), "tile_x must be 2, 4 or 8 (got %d)", tile_x);
+    TF_REQUIRE(tile_y >= 1, "tile_y must be >= 1");
+    std::lock_guard<std::mutex> lk(g_tiling_mu);
+    g_tiling[{nx, ny, nz}] = {tile_x, std::min(tile_y, ny)};
+    return TF_OK;
+}
+
+int tf_autotune_evolve(const float *in, float *out, int nx, int ny, int nz, double grid_spacing, double gamma,
+                       void *workspace, int *tile_x, int *tile_y, float *ms_per_step, void *stream) {
+    if (int e = check_dims(nx, ny, nz)) return e;
+    TF_REQUIRE(in && out && workspace && tile_x && tile_y, "null pointer");
+    TF_REQUIRE(in != out, "in and out must not alias");
+    hipStream_t st = S_(stream);
+    auto *H = reinterpret_cast<WsHeader *>(workspace);
+    double *dt = &H->totals[0];
+    // a realistic dt: the CFL step of the input
+    TF_HIP(launch_fill_u32(H->bits, 0u, 1, st));
+    TF_HIP(launch_speed(in, (int64_t)nx * ny * nz, (float)gamma, H->bits, st));
+    TF_HIP(launch_dt_from_bits(H->bits, grid_spacing, 0.4, dt, nullptr, st));
+    std::vector<int> lys;
+    for (int v : {8, 16, 32, 64, 128, 256, 512}) if (v < ny) lys.push_back(v);
+    lys.push_back(ny);
+    hipEvent_t e0, e1;
+    TF_HIP(hipEventCreate(&e0));
+    TF_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    int btx = 4, bly = ny;
+    int rc = TF_OK;
+    for (int tx : {2, 4, 8}) {
+        if (tx > 2 && tx > nx) continue;
+        for (int ly : lys) {
+            const SimDev S = make_simdev(nx, ny, nz, 0, gamma, tx, ly);
+            hipError_t e = launch_step_direct(tx, S, in, out, dt, grid_spacing, nullptr, st);
+            const int reps = 3;
+            if (e == hipSuccess) e = hipEventRecord(e0, st);
+            for (int r = 0; r < reps && e == hipSuccess; ++r)
+                e = launch_step_direct(tx, S, in, out, dt, grid_spacing, nullptr, st);
+            if (e == hipSuccess) e = hipEventRecord(e1, st);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float ms = 0.f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (e != hipSuccess) {
+                rc = fail(TF_ERR_HIP, "autotune launch: %s", hipGetErrorString(e));
+                break;
+            }
+            ms /= reps;
+            if (ms < best) {
+                best = ms;
+                btx = tx;
+                bly = S.ly;
+            }
+        }
+        if (rc != TF_OK) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != TF_OK) return rc;
+    {
+        std::lock_guard<std::mutex> lk(g_tiling_mu);
+        g_tiling[{nx, ny, nz}] = {btx, bly};
+    }
+    *tile_x = btx;
+    *tile_y = bly;
+    if (ms_per_step) *ms_per_step = best;
+    return TF_OK;
+}
+
+// ------------------------------------------------------------------ batched engine
+struct tf_batch {
+    int nsims = 0;
+    int tx = 4;
+    int total_waves = 0;
+    SimDev *d_sims = nullptr;
+    int *d_prefix = nullptr;
+    Ctrl *d_status = nullptr;
+    std::vector<SimDev> h_sims;
+    std::vector<int> h_prefix;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t capture_stream = nullptr;
+    int graph_steps = 0;
+};
+
+static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+size_t tf_batch_workspace_bytes(int nsims) {
+    if (nsims < 0) nsims = 0;
+    return align256((size_t)nsims * sizeof(SimDev)) + align256((size_t)(nsims + 1) * sizeof(int)) +
+           align256((size_t)nsims * sizeof(Ctrl));
+}
+
+int tf_sim_reset(const tf_sim_desc *sim, void *stream) {
+    TF_REQUIRE(sim, "null sim");
+    if (int e = check_dims(sim->nx, sim->ny, sim->nz)) return e;
+    TF_REQUIRE(sim->buf0 && sim->buf1 && sim->ctrl, "null device pointer in sim");
+    if (int e = check_open_bc(sim->handle_boundaries, sim->nx, sim->ny, sim->nz)) return e;
+    hipStream_t st = S_(stream);
+    Ctrl *C = reinterpret_cast<Ctrl *>(sim->ctrl);
+    TF_HIP(launch_ctrl_reset(C, sim->grid_spacing, sim->c_cfl, sim->t_final, sim->max_steps, st));
+    TF_HIP(launch_speed(sim->buf0, (int64_t)sim->nx * sim->ny * sim->nz, (float)sim->gamma, &C->max_bits[0], st));
+    TF_HIP(launch_finalize_one(C, 1, st));
+    return TF_OK;
+}
+
+int tf_batch_create(tf_batch **out, const tf_sim_desc *sims, int nsims, void *workspace, size_t workspace_bytes,
+                    int tile_x, int tile_y, void *stream) {
+    TF_REQUIRE(out, "null output handle");
+    *out = nullptr;
+    TF_REQUIRE(nsims >= 1, "nsims must be >= 1");
+    TF_REQUIRE(sims && workspace, "null pointer");
+    TF_REQUIRE(workspace_bytes >= tf_batch_workspace_bytes(nsims), "workspace too small: %zu < %zu",
+               workspace_bytes, tf_batch_workspace_bytes(nsims));
+    const int tx = tile_x == 0 ? 4 : tile_x;
+    TF_REQUIRE(valid_tx(tx), "tile_x must be 0, 2, 4 or 8 (got %d)", tile_x);
+    TF_REQUIRE(tile_y >= 0, "tile_y must be >= 0");
+    auto *b = new tf_batch();
+    b->nsims = nsims;
+    b->tx = tx;
+    b->h_sims.resize(nsims);
+    b->h_prefix.resize(nsims + 1);
+    b->h_prefix[0] = 0;
+    for (int i = 0; i < nsims; ++i) {
+        const tf_sim_desc &d = sims[i];
+        int rc = check_dims(d.nx, d.ny, d.nz);
+        if (rc == TF_OK) rc = check_open_bc(d.handle_boundaries, d.nx, d.ny, d.nz);
+        if (rc == TF_OK && !(d.buf0 && d.buf1 && d.ctrl)) rc = fail(TF_ERR_ARG, "null device pointer in sim %d", i);
+        if (rc != TF_OK) {
+            delete b;
+            return rc;
+        }
+        const int ly = tile_y > 0 ? tile_y : d.ny;  // tile_y == 0: refined below
+        SimDev S = make_simdev(d.nx, d.ny, d.nz, d.handle_boundaries, d.gamma, tx, ly);
+        S.buf[0] = d.buf0;
+        S.buf[1] = d.buf1;
+        S.ctrl = reinterpret_cast<Ctrl *>(d.ctrl);
+        b->h_sims[i] = S;
+        const long next = (long)b->h_prefix[i] + S.waves;
+        if (next > 2000000000L) {
+            delete b;
+            return fail(TF_ERR_ARG, "batch too large");
+        }
+        b->h_prefix[i + 1] = (int)next;
+    }
+    if (tile_y == 0) {
+        // Whole y-lines per wave unless the batch is too small to fill the
+        // chip; then halve the y-chunks of every simulation (down to 16 rows).
+        const long target = 256L * 4 * 3 * 6;
+        for (int ly_cap = 1 << 30; ; ly_cap /= 2) {
+            long total = 0;
+            for (int i = 0; i < nsims; ++i) {
+                const tf_sim_desc &d = sims[i];
+                const int ly = std::max(1, std::min(d.ny, std::max(16, ly_cap)));
+                b->h_sims[i] = [&] {
+                    SimDev S = make_simdev(d.nx, d.ny, d.nz, d.handle_boundaries, d.gamma, tx, ly);
+                    S.buf[0] = d.buf0;
+                    S.buf[1] = d.buf1;
+                    S.ctrl = reinterpret_cast<Ctrl *>(d.ctrl);
+                    return S;
+                }();
+                total += b->h_sims[i].waves;
+                b->h_prefix[i + 1] = (int)std::min(total, 2000000000L);
+            }
+            if (total >= target || ly_cap <= 16) break;
+        }
+    }
+    b->total_waves = b->h_prefix[nsims];
+    char *p = static_cast<char *>(workspace);
+    b->d_sims = reinterpret_cast<SimDev *>(p);
+    p += align256((size_t)nsims * sizeof(SimDev));
+    b->d_prefix = reinterpret_cast<int *>(p);
+    p += align256((size_t)(nsims + 1) * sizeof(int));
+    b->d_status = reinterpret_cast<Ctrl *>(p);
+    hipStream_t st = S_(stream);
+    hipError_t e = hipMemcpyAsync(b->d_sims, b->h_sims.data(), nsims * sizeof(SimDev), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->d_prefix, b->h_prefix.data(), (nsims + 1) * sizeof(int), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        delete b;
+        return fail(TF_ERR_HIP, "batch table upload: %s", hipGetErrorString(e));
+    }
+    *out = b;
+    return TF_OK;
+}
+
+int tf_batch_destroy(tf_batch *b) {
+    if (!b) return TF_OK;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
+    delete b;
+    return TF_OK;
+}
+
+static int enqueue_steps(tf_batch *b, int steps, hipStream_t st) {
+    for (int i = 0; i < steps; ++i) {
+        TF_HIP(launch_step_sims(b->tx, b->d_sims, b->d_prefix, b->nsims, b->total_waves, st));
+        TF_HIP(launch_finalize_sims(b->d_sims, b->nsims, st));
+    }
+    return TF_OK;
+}
+
+int tf_batch_step(tf_batch *b, int steps, void *stream) {
+    TF_REQUIRE(b, "null batch");
+    TF_REQUIRE(steps >= 0, "steps must be >= 0");
+    return enqueue_steps(b, steps, S_(stream));
+}
+
+int tf_batch_step_graph(tf_batch *b, int steps, void *stream) {
+    TF_REQUIRE(b, "null batch");
+    TF_REQUIRE(steps >= 1, "steps must be >= 1");
+    if (!b->exec || b->graph_steps != steps) {
+        if (b->exec) {
+            (void)hipGraphExecDestroy(b->exec);
+            b->exec = nullptr;
+        }
+        if (b->graph) {
+            (void)hipGraphDestroy(b->graph);
+            b->graph = nullptr;
+        }
+        if (!b->capture_stream) TF_HIP(hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking));
+        TF_HIP(hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_steps(b, steps, b->capture_stream);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(b->capture_stream, &g);
+        if (rc != TF_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(TF_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+        b->graph = g;
+        TF_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+        b->graph_steps = steps;
+    }
+    TF_HIP(hipGraphLaunch(b->exec, S_(stream)));
+    return TF_OK;
+}
+
+int tf_batch_step_timed(tf_batch *b, int steps, float *stencil_ms, void *stream) {
+    TF_REQUIRE(b && stencil_ms, "null pointer");
+    TF_REQUIRE(steps >= 1, "steps must be >= 1");
+    hipStream_t st = S_(stream);
+    std::vector<hipEvent_t> ev(2 * (size_t)steps, nullptr);
+    int rc = TF_OK;
+    for (auto &e : ev) {
+        if (hipEventCreate(&e) != hipSuccess) {
+            rc = fail(TF_ERR_HIP, "hipEventCreate failed");
+            break;
+        }
+    }
+    for (int i = 0; i < steps && rc == TF_OK; ++i) {
+        hipError_t e = hipEventRecord(ev[2 * i], st);
+        if (e == hipSuccess) e = launch_step_sims(b->tx, b->d_sims, b->d_prefix, b->nsims, b->total_waves, st);
+        if (e == hipSuccess) e = hipEventRecord(ev[2 * i + 1], st);
+        if (e == hipSuccess) e = launch_finalize_sims(b->d_sims, b->nsims, st);
+        if (e != hipSuccess) rc = fail(TF_ERR_HIP, "timed step: %s", hipGetErrorString(e));
+    }
+    float total = 0.f;
+    if (rc == TF_OK) {
+        hipError_t e = hipStreamSynchronize(st);
+        for (int i = 0; i < steps && e == hipSuccess; ++i) {
+            float ms = 0.f;
+            e = hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]);
+            total += ms;
+        }
+        if (e != hipSuccess) rc = fail(TF_ERR_HIP, "timed step events: %s", hipGetErrorString(e));
+    }
+    for (auto &e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (rc == TF_OK) *stencil_ms = total;
+    return rc;
+}
+
+int tf_batch_status(tf_batch *b, tf_sim_status *host_out, void *stream) {
+    TF_REQUIRE(b && host_out, "null pointer");
+    hipStream_t st = S_(stream);
+    TF_HIP(launch_gather_status(b->d_sims, b->nsims, b->d_status, st));
+    std::vector<Ctrl> tmp(b->nsims);
+    TF_HIP(hipMemcpyAsync(tmp.data(), b->d_status, b->nsims * sizeof(Ctrl), hipMemcpyDeviceToHost, st));
+    TF_HIP(hipStreamSynchronize(st));
+    for (int i = 0; i < b->nsims; ++i) std::memcpy(&host_out[i], &tmp[i], sizeof(tf_sim_status));
+    return TF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,146 @@
+// Device-side arithmetic of the HLL / Godunov step for gfx950.
+//
+// One cell of a sweep along axis D (1 = x, 2 = y, 3 = z) is described by its
+// conserved vector U, its physical flux F along D, and its two characteristic
+// speeds u_D + c and u_D - c. The Davis wave speeds, HLL flux and Godunov
+// update below are the reference's fluid.py:119-160, :206-305 and :506-557,
+// rearranged so that every per-cell quantity is computed once per sweep:
+//   S_R = max(u_L + c_L, u_R + c_R, 0),  S_L = min(u_L - c_L, u_R - c_R, 0)
+//   F   = (S_R F_L - S_L F_R + S_L S_R (U_R - U_L)) / (S_R - S_L)
+//   U' = U + dt/dx (F[i-1/2] - F[i+1/2])
+// with one v_rcp_f32 per interface and per cell in place of IEEE division
+// (1 ulp), c = sqrt(gamma p / rho) via v_sqrt_f32, and the kinetic energy as
+// m.v instead of rho*sqrt(v.v)^2.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tf {
+
+struct Gas {
+    float gamma;    // adiabatic index
+    float gm1;      // gamma - 1
+    float inv_gm1;  // 1 / (gamma - 1)
+    float k;        // dt / dx of the current step
+};
+
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float fmax3(float a, float b, float c) { return fmaxf(fmaxf(a, b), c); }
+__device__ __forceinline__ float fmin3(float a, float b, float c) { return fminf(fminf(a, b), c); }
+
+// Primitive cell with 1/rho carried along (rho, vx, vy, vz, p, rinv).
+struct Prim {
+    float r, v[3], p, rinv;
+};
+
+// One cell of a sweep: conserved U, flux F along the sweep axis, u_D +- c.
+struct Q {
+    float U[5];
+    float F[5];
+    float up, um;
+};
+
+// Primitive -> conserved for a loaded cell (fluid.py:28-67), also 1/rho.
+__device__ __forceinline__ void prim_to_cons(const float w[5], const Gas &g, float U[5], float &rinv) {
+    const float r = w[0];
+    U[0] = r;
+    U[1] = r * w[1];
+    U[2] = r * w[2];
+    U[3] = r * w[3];
+    const float mv = fmaf(U[3], w[3], fmaf(U[2], w[2], U[1] * w[1]));
+    U[4] = fmaf(0.5f, mv, w[4] * g.inv_gm1);
+    rinv = frcp(r);
+}
+
+// Conserved -> primitive (fluid.py:69-103).
+__device__ __forceinline__ Prim cons_to_prim(const float U[5], const Gas &g) {
+    Prim w;
+    w.r = U[0];
+    w.rinv = frcp(U[0]);
+    w.v[0] = U[1] * w.rinv;
+    w.v[1] = U[2] * w.rinv;
+    w.v[2] = U[3] * w.rinv;
+    const float mv = fmaf(U[3], w.v[2], fmaf(U[2], w.v[1], U[1] * w.v[0]));
+    w.p = g.gm1 * fmaf(-0.5f, mv, U[4]);
+    return w;
+}
+
+// Sound speed (fluid.py:105-117) from p and 1/rho.
+__device__ __forceinline__ float sound(float p, float rinv, const Gas &g) { return fsqrt(g.gamma * p * rinv); }
+
+// Sweep quantities along axis D (1..3) of a cell with conserved U and
+// primitive velocity/pressure (fluid.py:206-259 for F).
+template <int D>
+__device__ __forceinline__ Q make_q(const float U[5], const float v[3], float p, float rinv, const Gas &g) {
+    Q q;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) q.U[i] = U[i];
+    const float u = v[D - 1];
+    q.F[0] = U[D];
+    q.F[1] = U[1] * u;
+    q.F[2] = U[2] * u;
+    q.F[3] = U[3] * u;
+    q.F[D] = fmaf(U[D], u, p);
+    q.F[4] = (U[4] + p) * u;
+    const float c = sound(p, rinv, g);
+    q.up = u + c;
+    q.um = u - c;
+    return q;
+}
+
+// HLL flux at the interface between L and R (fluid.py:261-305).
+__device__ __forceinline__ void hll(const Q &L, const Q &R, float F[5]) {
+    const float sr = fmax3(L.up, R.up, 0.0f);
+    const float sl = fmin3(L.um, R.um, 0.0f);
+    const float inv = frcp(sr - sl);
+    const float a = sr * inv;
+    const float b = sl * inv;
+    const float e = a * sl;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) F[i] = fmaf(e, R.U[i] - L.U[i], fmaf(a, L.F[i], -b * R.F[i]));
+}
+
+// Godunov update U + k (F_left - F_right) (fluid.py:547-551).
+__device__ __forceinline__ void godunov(const float U[5], const float Fl[5], const float Fr[5], float k,
+                                        float out[5]) {
+#pragma unroll
+    for (int i = 0; i < 5; ++i) out[i] = fmaf(k, Fl[i] - Fr[i], U[i]);
+}
+
+// max_d |v_d| + c: the cell's contribution to the CFL signal speed. It equals
+// the max over the cell's interfaces of max(|S_R|, |S_L|) of fluid.py:162-195
+// because rounding is monotone and sign-symmetric.
+__device__ __forceinline__ float signal_speed(const float v[3], float c) {
+    return fmax3(fabsf(v[0]), fabsf(v[1]), fabsf(v[2])) + c;
+}
+
+// Periodic wrap of i in [-n, 2n) and the open-boundary index clamp of
+// fluid.py:382-413 (copies from the inner neighbour, applied after the wrap).
+__device__ __forceinline__ int wrap1(int i, int n) {
+    i = i < 0 ? i + n : i;
+    return i >= n ? i - n : i;
+}
+__device__ __forceinline__ int open_clamp(int i, int n) { return n == 2 ? 1 : min(max(i, 1), n - 2); }
+__device__ __forceinline__ int cell_index(int i, int n, int open_bc) {
+    int j = i % n;
+    j = j < 0 ? j + n : j;
+    return open_bc ? open_clamp(j, n) : j;
+}
+
+// Lane shifts inside a wave64 (DPP): from_left gives lane l the value of
+// lane l-1, from_right the value of lane l+1 (edge lanes receive 0).
+__device__ __forceinline__ float from_left(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float from_right(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xf, 0xf, false));
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+}  // namespace tf

@@ -1,0 +1,58 @@
+// Internal host/device structures shared by the kernels and the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "tf_device.h"
+
+namespace tf {
+
+constexpr int kWave = 64;
+constexpr int kZT = kWave - 2;      // output z-columns per wave (lanes 1..62; lanes 0 and 63 are halo)
+constexpr int kBlock = 256;         // threads per block of the stencil kernels (4 independent waves)
+constexpr int kTotalsBlocks = 1024; // fixed grid of the float64 totals reduction (deterministic order)
+
+// Per-simulation control block in device memory. The first 48 bytes mirror
+// tf_sim_status of the public header.
+struct Ctrl {
+    double t;           // simulated time of the current state
+    double dt;          // dt of the next step
+    int64_t n_iter;     // completed steps; the state is in buf[n_iter & 1]
+    int32_t active;     // the next step runs
+    int32_t nonfinite;  // a non-finite signal speed was seen
+    float max_speed;    // max |v_d| + c of the current state
+    int32_t pad0[3];
+    uint32_t max_bits[2];  // signal-speed accumulators (float bits), by parity of the state written
+    float k;               // float(dt / dx)
+    int32_t pad1;
+    double t_final;
+    double dx;
+    double c_cfl;
+    int64_t max_steps;     // < 0: unlimited
+    double pad2[4];
+};
+static_assert(sizeof(Ctrl) == 128, "Ctrl must match TF_CTRL_BYTES");
+
+// Geometry and physics of one simulation as the stencil kernels see it.
+struct SimDev {
+    float *buf[2];
+    Ctrl *ctrl;
+    int nx, ny, nz, open_bc;
+    float gamma, gm1, inv_gm1, pad;
+    int ntx, ntz, nyc, ly;  // tiles along x (of TX), z (of kZT), y-chunks and their length
+    int waves;              // ntx * ntz * nyc
+    int pad2[3];
+};
+
+// Workspace header of the single-grid entry points.
+struct WsHeader {
+    SimDev sim;
+    Ctrl ctrl;
+    int prefix[2];
+    uint32_t bits[4];
+    double totals[5];
+};
+
+constexpr size_t kWsHeaderBytes = 1024;
+static_assert(sizeof(WsHeader) <= kWsHeaderBytes, "workspace header too large");
+
+}  // namespace tf
