@@ -36,6 +36,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--tile", type=str, default="", help="force tile_x,tile_y (skips autotuning)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also copy the state buffer once (a plain copy kernel of known size, for HBM counters)")
     return ap.parse_args()
 
 
@@ -99,7 +102,10 @@ def main():
     import ctypes
     tx, ly = ctypes.c_int(0), ctypes.c_int(0)
     tuned_ms = None
-    if not args.no_tune:
+    if args.tile:
+        tx.value, ly.value = (int(v) for v in args.tile.split(","))
+        _lib.check(lib.tf_set_tiling(nx, ny, nz, tx.value, ly.value))
+    elif not args.no_tune:
         ws = tf.fluid.workspace(nx, ny, nz, dev)
         ms = ctypes.c_float(0)
         _lib.check(lib.tf_autotune_evolve(buf0.data_ptr(), buf1.data_ptr(), nx, ny, nz, dx, gamma, ws.data_ptr(),
@@ -110,6 +116,10 @@ def main():
         _lib.check(lib.tf_get_tiling(nx, ny, nz, ctypes.byref(tx), ctypes.byref(ly)))
     print(f"bench: tiling tile_x={tx.value} tile_y={ly.value} (tuned {tuned_ms} ms)", file=sys.stderr, flush=True)
 
+    if args.calibrate_copy:
+        buf1.copy_(buf0)
+        torch.cuda.synchronize()
+        print(f"bench: calibration copy of {buf0.numel() * 4} bytes", file=sys.stderr, flush=True)
     sim = Simulation(buf0, dx, gamma, buffers=(buf0, buf1))
     batch = Batch([sim], tile_x=tx.value, tile_y=ly.value)
     sim.reset()

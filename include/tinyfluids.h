@@ -154,7 +154,7 @@ typedef struct tf_batch tf_batch;
 size_t tf_batch_workspace_bytes(int nsims);
 
 /* Builds a plan; its device tables live in the caller's workspace.
- * tile_x (x-cells per wave: 2, 4 or 8) and tile_y (rows per y-chunk) choose
+ * tile_x (x-cells per wave: 2, 4, 6 or 8) and tile_y (rows per y-chunk) choose
  * the stencil tiling; 0 picks the library default. */
 int tf_batch_create(tf_batch **out, const tf_sim_desc *sims, int nsims, void *workspace, size_t workspace_bytes,
                     int tile_x, int tile_y, void *stream);

@@ -13,7 +13,8 @@ import subprocess
 import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libtfhip.so")
+# TINYFLUIDS_LIB selects another build of the same library (A/B timing).
+LIB_PATH = os.environ.get("TINYFLUIDS_LIB") or os.path.join(_HERE, "libtfhip.so")
 _CSRC = os.path.join(_HERE, "csrc")
 
 TF_OK = 0

@@ -89,7 +89,7 @@ Gas make_gas(double gamma, float k = 0.0f) {
 std::mutex g_tiling_mu;
 std::map<std::tuple<int, int, int>, std::pair<int, int>> g_tiling;
 
-int default_tile_x(int nx) { return nx >= 4 ? 4 : 2; }
+int default_tile_x(int nx) { return nx >= 8 ? 8 : (nx >= 4 ? 4 : 2); }
 
 // Rows per y-chunk: enough waves to fill the chip (256 CUs, a few waves per
 // SIMD) with several rounds so the last round is short, without letting the
@@ -136,7 +136,14 @@ SimDev make_simdev(int nx, int ny, int nz, int open_bc, double gamma, int tx, in
     return S;
 }
 
-bool valid_tx(int tx) { return tx == 2 || tx == 4 || tx == 8; }
+bool valid_tx(int tx) { return tx == 2 || tx == 4 || tx == 6 || tx == 8; }
+
+// The fused stencil addresses each field with 32-bit byte offsets.
+int check_fused_size(int nx, int ny, int nz) {
+    if ((double)nx * ny * nz > 1073741824.0)
+        return fail(TF_ERR_UNSUPPORTED, "fused step: %d x %d x %d exceeds 2^30 cells per field", nx, ny, nz);
+    return TF_OK;
+}
 
 int check_open_bc(int open_bc, int nx, int ny, int nz) {
     if (open_bc) TF_REQUIRE(nx >= 2 && ny >= 2 && nz >= 2, "open boundaries need every dimension >= 2");
@@ -211,6 +218,7 @@ int tf_evolve_state(const float *in, float *out, int nx, int ny, int nz, double 
                     void *stream) {
     (void)workspace;
     if (int e = check_dims(nx, ny, nz)) return e;
+    if (int e = check_fused_size(nx, ny, nz)) return e;
     TF_REQUIRE(in && out && dt, "null pointer");
     TF_REQUIRE(in != out, "in and out must not alias");
     if (int e = check_open_bc(handle_boundaries, nx, ny, nz)) return e;
@@ -226,6 +234,7 @@ int tf_time_integration(float *state, float *scratch, int nx, int ny, int nz, do
                         int64_t *num_iterations, double *t_out, int *result_in_scratch, int *nonfinite,
                         void *stream) {
     if (int e = check_dims(nx, ny, nz)) return e;
+    if (int e = check_fused_size(nx, ny, nz)) return e;
     TF_REQUIRE(state && scratch && workspace, "null device pointer");
     TF_REQUIRE(num_iterations && t_out && result_in_scratch, "null host result pointer");
     TF_REQUIRE(state != scratch, "state and scratch must be different buffers");
@@ -284,7 +293,7 @@ int tf_get_tiling(int nx, int ny, int nz, int *tile_x, int *tile_y) {
 
 int tf_set_tiling(int nx, int ny, int nz, int tile_x, int tile_y) {
     if (int e = check_dims(nx, ny, nz)) return e;
-    TF_REQUIRE(valid_tx(tile_x), "tile_x must be 2, 4 or 8 (got %d)", tile_x);
+    TF_REQUIRE(valid_tx(tile_x), "tile_x must be 2, 4, 6 or 8 (got %d)", tile_x);
     TF_REQUIRE(tile_y >= 1, "tile_y must be >= 1");
     std::lock_guard<std::mutex> lk(g_tiling_mu);
     g_tiling[{nx, ny, nz}] = {tile_x, std::min(tile_y, ny)};
@@ -294,6 +303,7 @@ int tf_set_tiling(int nx, int ny, int nz, int tile_x, int tile_y) {
 int tf_autotune_evolve(const float *in, float *out, int nx, int ny, int nz, double grid_spacing, double gamma,
                        void *workspace, int *tile_x, int *tile_y, float *ms_per_step, void *stream) {
     if (int e = check_dims(nx, ny, nz)) return e;
+    if (int e = check_fused_size(nx, ny, nz)) return e;
     TF_REQUIRE(in && out && workspace && tile_x && tile_y, "null pointer");
     TF_REQUIRE(in != out, "in and out must not alias");
     hipStream_t st = S_(stream);
@@ -312,7 +322,7 @@ int tf_autotune_evolve(const float *in, float *out, int nx, int ny, int nz, doub
     float best = 1e30f;
     int btx = 4, bly = ny;
     int rc = TF_OK;
-    for (int tx : {2, 4, 8}) {
+    for (int tx : {2, 4, 6, 8}) {
         if (tx > 2 && tx > nx) continue;
         for (int ly : lys) {
             const SimDev S = make_simdev(nx, ny, nz, 0, gamma, tx, ly);
@@ -397,7 +407,7 @@ int tf_batch_create(tf_batch **out, const tf_sim_desc *sims, int nsims, void *wo
     TF_REQUIRE(workspace_bytes >= tf_batch_workspace_bytes(nsims), "workspace too small: %zu < %zu",
                workspace_bytes, tf_batch_workspace_bytes(nsims));
     const int tx = tile_x == 0 ? 4 : tile_x;
-    TF_REQUIRE(valid_tx(tx), "tile_x must be 0, 2, 4 or 8 (got %d)", tile_x);
+    TF_REQUIRE(valid_tx(tx), "tile_x must be 0, 2, 4, 6 or 8 (got %d)", tile_x);
     TF_REQUIRE(tile_y >= 0, "tile_y must be >= 0");
     auto *b = new tf_batch();
     b->nsims = nsims;
@@ -408,6 +418,7 @@ int tf_batch_create(tf_batch **out, const tf_sim_desc *sims, int nsims, void *wo
     for (int i = 0; i < nsims; ++i) {
         const tf_sim_desc &d = sims[i];
         int rc = check_dims(d.nx, d.ny, d.nz);
+        if (rc == TF_OK) rc = check_fused_size(d.nx, d.ny, d.nz);
         if (rc == TF_OK) rc = check_open_bc(d.handle_boundaries, d.nx, d.ny, d.nz);
         if (rc == TF_OK && !(d.buf0 && d.buf1 && d.ctrl)) rc = fail(TF_ERR_ARG, "null device pointer in sim %d", i);
         if (rc != TF_OK) {

@@ -89,6 +89,40 @@ __device__ __forceinline__ Q make_q(const float U[5], const float v[3], float p,
     return q;
 }
 
+// Compact form of Q kept between rows of a stream: U, the velocity along the
+// sweep axis, p and c; F and u +- c are rebuilt when needed.
+struct QS {
+    float U[5];
+    float u, p, c;
+};
+
+template <int D>
+__device__ __forceinline__ QS make_qs(const float U[5], const float v[3], float p, float rinv, const Gas &g) {
+    QS q;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) q.U[i] = U[i];
+    q.u = v[D - 1];
+    q.p = p;
+    q.c = sound(p, rinv, g);
+    return q;
+}
+
+template <int D>
+__device__ __forceinline__ Q expand_qs(const QS &s) {
+    Q q;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) q.U[i] = s.U[i];
+    q.F[0] = s.U[D];
+    q.F[1] = s.U[1] * s.u;
+    q.F[2] = s.U[2] * s.u;
+    q.F[3] = s.U[3] * s.u;
+    q.F[D] = fmaf(s.U[D], s.u, s.p);
+    q.F[4] = (s.U[4] + s.p) * s.u;
+    q.up = s.u + s.c;
+    q.um = s.u - s.c;
+    return q;
+}
+
 // HLL flux at the interface between L and R (fluid.py:261-305).
 __device__ __forceinline__ void hll(const Q &L, const Q &R, float F[5]) {
     const float sr = fmax3(L.up, R.up, 0.0f);
@@ -128,13 +162,14 @@ __device__ __forceinline__ int cell_index(int i, int n, int open_bc) {
     return open_bc ? open_clamp(j, n) : j;
 }
 
-// Lane shifts inside a wave64 (DPP): from_left gives lane l the value of
-// lane l-1, from_right the value of lane l+1 (edge lanes receive 0).
+// Lane shifts inside a wave64 (DPP wave_shr:1 / wave_shl:1): from_left gives
+// lane l the value of lane l-1, from_right the value of lane l+1; the edge
+// lane with no source reads 0 (bound_ctrl).
 __device__ __forceinline__ float from_left(float v) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xf, 0xf, false));
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x138, 0xf, 0xf, true));
 }
 __device__ __forceinline__ float from_right(float v) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xf, 0xf, false));
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x130, 0xf, 0xf, true));
 }
 
 __device__ __forceinline__ float wave_max(float v) {

@@ -25,6 +25,14 @@
 // the halo rows and columns are re-read from L2 / MALL.
 #include "tf_internal.h"
 
+// Cache policy of the state stores. Plain write-back stores (0) let L2 merge
+// the partial 128-byte lines that two neighbouring z-tiles write (a tile's
+// 62 columns are not line-aligned); streaming (NT) stores write those halves
+// to HBM separately and measured ~25% slower.
+#ifndef TF_STORE_AUX
+#define TF_STORE_AUX 0
+#endif
+
 namespace tf {
 
 __device__ __forceinline__ int wave_id_uniform() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
@@ -36,16 +44,11 @@ __device__ __forceinline__ uint32_t wave_umax(uint32_t v) {
     return v;
 }
 
-template <int TX>
-__device__ __forceinline__ void load_row(const float *__restrict__ in, size_t nc, const size_t (&xoff)[TX + 2],
-                                         size_t yoff, uint32_t zg, float (&dst)[TX + 2][5]) {
-#pragma unroll
-    for (int j = 0; j < TX + 2; ++j) {
-        const float *row = in + xoff[j] + yoff;
-#pragma unroll
-        for (int f = 0; f < 5; ++f) dst[j][f] = __builtin_nontemporal_load(row + f * nc + zg);
-    }
-}
+// Global-address-space views of the state buffers: loads and stores then use
+// a scalar (SGPR) row base plus one 32-bit per-lane offset, with no per-lane
+// 64-bit address arithmetic.
+typedef __attribute__((address_space(1))) const float gcfloat;
+typedef __attribute__((address_space(1))) float gfloat;
 
 // Sweep quantities along x of a loaded primitive cell.
 __device__ __forceinline__ Q q_from_prim_x(const float (&w)[5], const Gas &g) {
@@ -55,123 +58,241 @@ __device__ __forceinline__ Q q_from_prim_x(const float (&w)[5], const Gas &g) {
     return make_q<1>(U, v, w[4], rinv, g);
 }
 
+// Loads the 5 fields of one cell: `col` is the wave-uniform address of the
+// cell row's first field, `zb` the lane's byte offset along z (< 4 GiB), so
+// each load is a global_load with an SGPR base and a 32-bit VGPR offset.
+typedef __attribute__((address_space(1))) const char gcbyte;
+typedef __attribute__((address_space(1))) char gbyte;
+
+// A wave-uniform address pinned to SGPRs: the compiler cannot fold a lane
+// offset into it, so base + 32-bit lane offset selects the SGPR-base form.
+template <typename T>
+__device__ __forceinline__ T *sgpr_ptr(T *p) {
+    const uint64_t v = (uint64_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return (T *)(((uint64_t)hi << 32) | lo);
+}
+
+// State access of one tile. Each of the 5 fields of the input and output
+// has a buffer resource in SGPRs; a cell row is a 32-bit scalar byte offset
+// (x/y position) plus the lane's byte offset along z, so loads and stores
+// need no per-lane address arithmetic. Fields must be < 4 GiB
+// (nx*ny*nz <= 2^30); larger grids are rejected on the host.
+template <int TX>
+struct TileCtx {
+    __amdgpu_buffer_rsrc_t rin[5], rout[5];
+    uint32_t xoff[TX + 2];  // byte offsets of the loaded x-rows (uniform)
+    uint32_t zb;            // this lane's load byte offset along z
+    uint32_t zbo;           // this lane's store byte offset (z - z0)
+    int nz, ny, y0, x0, z0, nxo, nrows, open_bc;
+    uint32_t plane_b, row_b;  // bytes per x-plane and per y-row of one field
+    bool zout;
+};
+
+__device__ __forceinline__ float buf_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, (int)voff, (int)soff, TF_STORE_AUX);
+}
+
+template <int TX>
+__device__ __forceinline__ void load_cell(const TileCtx<TX> &T, uint32_t soff, float (&w)[5]) {
+#pragma unroll
+    for (int f = 0; f < 5; ++f) w[f] = buf_load(T.rin[f], T.zb, soff);
+}
+
+// One row of the stream: X sweep of row y0-1+r, its Y flux against the
+// previous row (qa/fa in), the Y update + Z sweep + store of the previous
+// row, and the reload of every loaded column with row r+1. Writes the new
+// Y-stream state to qb/fb.
+template <int TX>
+__device__ __forceinline__ void stream_row(const TileCtx<TX> &T, const Gas &g, int r, float (&cur)[TX + 2][5],
+                                           const QS (&qa)[TX], const float (&fa)[TX][5], QS (&qb)[TX],
+                                           float (&fb)[TX][5], uint32_t &vmax) {
+    const int yn = r + 1 < T.nrows ? T.y0 + r : T.y0 + r - 1;  // the last row reloads itself
+    const int ygn = T.open_bc ? open_clamp(wrap1(yn, T.ny), T.ny) : wrap1(yn, T.ny);
+    const uint32_t yoffn = (uint32_t)__builtin_amdgcn_readfirstlane(ygn) * T.row_b;
+    Q qx0 = q_from_prim_x(cur[0], g);
+    load_cell(T, T.xoff[0] + yoffn, cur[0]);
+    Q qx1 = q_from_prim_x(cur[1], g);
+    load_cell(T, T.xoff[1] + yoffn, cur[1]);
+    float fx0[5];
+    hll(qx0, qx1, fx0);
+#pragma unroll
+    for (int s = 0; s < TX; ++s) {
+        const Q qx2 = q_from_prim_x(cur[s + 2], g);
+        load_cell(T, T.xoff[s + 2] + yoffn, cur[s + 2]);
+        float fx1[5];
+        hll(qx1, qx2, fx1);
+        float U1[5];
+        godunov(qx1.U, fx0, fx1, g.k, U1);
+        const Prim w1 = cons_to_prim(U1, g);
+        qb[s] = make_qs<2>(U1, w1.v, w1.p, w1.rinv, g);
+        if (r >= 1) hll(expand_qs<2>(qa[s]), expand_qs<2>(qb[s]), fb[s]);  // Y flux: previous row | this row
+        if (r >= 2) {
+            float U2[5];
+            godunov(qa[s].U, fa[s], fb[s], g.k, U2);  // the previous row is now Y-swept
+            const Prim w2 = cons_to_prim(U2, g);
+            const Q q2 = make_q<3>(U2, w2.v, w2.p, w2.rinv, g);
+            Q ql;
+#pragma unroll
+            for (int i = 0; i < 5; ++i) {
+                ql.U[i] = from_left(q2.U[i]);
+                ql.F[i] = from_left(q2.F[i]);
+            }
+            ql.up = from_left(q2.up);
+            ql.um = from_left(q2.um);
+            float fz[5], fzr[5];
+            hll(ql, q2, fz);
+#pragma unroll
+            for (int i = 0; i < 5; ++i) fzr[i] = from_right(fz[i]);
+            float U3[5];
+            godunov(q2.U, fz, fzr, g.k, U3);
+            const Prim w3 = cons_to_prim(U3, g);
+            if (T.zout && s < T.nxo) {
+                const uint32_t so = (uint32_t)(T.x0 + s) * T.plane_b + (uint32_t)(T.y0 + r - 2) * T.row_b +
+                                    4u * (uint32_t)T.z0;
+                buf_store(T.rout[0], T.zbo, so, w3.r);
+                buf_store(T.rout[1], T.zbo, so, w3.v[0]);
+                buf_store(T.rout[2], T.zbo, so, w3.v[1]);
+                buf_store(T.rout[3], T.zbo, so, w3.v[2]);
+                buf_store(T.rout[4], T.zbo, so, w3.p);
+                const float sp = signal_speed(w3.v, sound(w3.p, w3.rinv, g));
+                vmax = max(vmax, __float_as_uint(sp));
+            }
+        }
+        qx1 = qx2;
+#pragma unroll
+        for (int i = 0; i < 5; ++i) fx0[i] = fx1[i];
+    }
+}
+
 // One wave: the fused X->Y->Z step on its tile. Returns the wave-local max
 // of the signal-speed bits of the cells it wrote (0 if none).
+__device__ __forceinline__ float sgpr_f(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+
 template <int TX>
-__device__ __forceinline__ uint32_t fused_tile(const float *__restrict__ in, float *__restrict__ out, int nx, int ny,
-                                               int nz, int open_bc, const Gas &g, int xt, int zt, int yc, int ly) {
-    const int lane = lane_id();
-    const int x0 = xt * TX, z0 = zt * kZT, y0 = yc * ly;
-    const int nzo = min(kZT, nz - z0);
-    const int nyo = min(ly, ny - y0);
-    const int nxo = min(TX, nx - x0);
-    const size_t plane = (size_t)ny * nz;
-    const size_t nc = plane * nx;
-    const uint32_t zg = (uint32_t)cell_index(z0 - 1 + lane, nz, open_bc);
-    const bool zout = lane >= 1 && lane <= nzo;
-
-    size_t xoff[TX + 2];
+__device__ __forceinline__ uint32_t fused_tile(const float *in_, float *out_, int nx, int ny, int nz, int open_bc,
+                                               const Gas &g_in, int xt, int zt, int yc, int ly) {
+    const Gas g{sgpr_f(g_in.gamma), sgpr_f(g_in.gm1), sgpr_f(g_in.inv_gm1), sgpr_f(g_in.k)};
+    // Everything below is wave-uniform; readfirstlane keeps it in SGPRs even
+    // where it was read from memory with vector loads.
+    nx = __builtin_amdgcn_readfirstlane(nx);
+    ny = __builtin_amdgcn_readfirstlane(ny);
+    nz = __builtin_amdgcn_readfirstlane(nz);
+    open_bc = __builtin_amdgcn_readfirstlane(open_bc);
+    xt = __builtin_amdgcn_readfirstlane(xt);
+    zt = __builtin_amdgcn_readfirstlane(zt);
+    yc = __builtin_amdgcn_readfirstlane(yc);
+    ly = __builtin_amdgcn_readfirstlane(ly);
+    TileCtx<TX> T;
+    const size_t nc = (size_t)nx * ny * nz;
+    {
+        const float *pin = sgpr_ptr(in_);
+        float *pout = sgpr_ptr(out_);
 #pragma unroll
-    for (int j = 0; j < TX + 2; ++j) xoff[j] = (size_t)cell_index(x0 - 1 + j, nx, open_bc) * plane;
-
-    float cur[TX + 2][5];
-    load_row<TX>(in, nc, xoff, (size_t)cell_index(y0 - 1, ny, open_bc) * nz, zg, cur);
-
-    Q qy[TX];         // Y-sweep quantities of the previous row
-    float fy[TX][5];  // Y flux at the lower interface of the previous row
-    uint32_t vmax = 0;
-    const int nrows = nyo + 2;
-
-    for (int r = 0; r < nrows; ++r) {
-        float nxt[TX + 2][5];
-        if (r + 1 < nrows) {
-            const int yn = y0 + r;  // row y0 - 1 + (r + 1)
-            const int ygn = open_bc ? cell_index(yn, ny, 1) : wrap1(yn, ny);
-            load_row<TX>(in, nc, xoff, (size_t)ygn * nz, zg, nxt);
+        for (int f = 0; f < 5; ++f) {
+            T.rin[f] = __builtin_amdgcn_make_buffer_rsrc((void *)(pin + f * nc), (short)0, -1, 0x00020000);
+            T.rout[f] = __builtin_amdgcn_make_buffer_rsrc((void *)(pout + f * nc), (short)0, -1, 0x00020000);
         }
-        // ---- X sweep of this row: cells j = 1..TX from x-neighbours j-1, j+1
-        Q qb = q_from_prim_x(cur[0], g);
-        Q qc = q_from_prim_x(cur[1], g);
-        float fa[5];
-        hll(qb, qc, fa);
-#pragma unroll
-        for (int s = 0; s < TX; ++s) {
-            const Q qn = q_from_prim_x(cur[s + 2], g);
-            float fb[5];
-            hll(qc, qn, fb);
-            float U1[5];
-            godunov(qc.U, fa, fb, g.k, U1);
-            const Prim w1 = cons_to_prim(U1, g);
-            const Q q1 = make_q<2>(U1, w1.v, w1.p, w1.rinv, g);
-            // ---- Y sweep: interface between the previous row and this one
-            if (r == 0) {
-                qy[s] = q1;
-            } else if (r == 1) {
-                hll(qy[s], q1, fy[s]);
-                qy[s] = q1;
-            } else {
-                float fn[5];
-                hll(qy[s], q1, fn);
-                float U2[5];
-                godunov(qy[s].U, fy[s], fn, g.k, U2);  // the previous row is now Y-swept
-#pragma unroll
-                for (int i = 0; i < 5; ++i) fy[s][i] = fn[i];
-                qy[s] = q1;
-                // ---- Z sweep across lanes
-                const Prim w2 = cons_to_prim(U2, g);
-                const Q q2 = make_q<3>(U2, w2.v, w2.p, w2.rinv, g);
-                Q ql;
-#pragma unroll
-                for (int i = 0; i < 5; ++i) {
-                    ql.U[i] = from_left(q2.U[i]);
-                    ql.F[i] = from_left(q2.F[i]);
-                }
-                ql.up = from_left(q2.up);
-                ql.um = from_left(q2.um);
-                float fz[5], fzr[5];
-                hll(ql, q2, fz);
-#pragma unroll
-                for (int i = 0; i < 5; ++i) fzr[i] = from_right(fz[i]);
-                float U3[5];
-                godunov(q2.U, fz, fzr, g.k, U3);
-                const Prim w3 = cons_to_prim(U3, g);
-                if (zout && s < nxo) {
-                    float *o = out + (size_t)(x0 + s) * plane + (size_t)(y0 + r - 2) * nz + (uint32_t)(z0 - 1 + lane);
-                    __builtin_nontemporal_store(w3.r, o);
-                    __builtin_nontemporal_store(w3.v[0], o + nc);
-                    __builtin_nontemporal_store(w3.v[1], o + 2 * nc);
-                    __builtin_nontemporal_store(w3.v[2], o + 3 * nc);
-                    __builtin_nontemporal_store(w3.p, o + 4 * nc);
-                    const float sp = signal_speed(w3.v, sound(w3.p, w3.rinv, g));
-                    vmax = max(vmax, __float_as_uint(sp));
-                }
-            }
-            qc = qn;
-#pragma unroll
-            for (int i = 0; i < 5; ++i) fa[i] = fb[i];
-        }
-#pragma unroll
-        for (int j = 0; j < TX + 2; ++j)
-#pragma unroll
-            for (int f = 0; f < 5; ++f) cur[j][f] = nxt[j][f];
     }
+    const int lane = lane_id();
+    T.x0 = xt * TX;
+    T.z0 = zt * kZT;
+    T.y0 = yc * ly;
+    T.ny = ny;
+    T.nz = nz;
+    T.open_bc = open_bc;
+    const int nzo = min(kZT, nz - T.z0);
+    T.nrows = min(ly, ny - T.y0) + 2;
+    T.nxo = min(TX, nx - T.x0);
+    T.row_b = 4u * (uint32_t)nz;
+    T.plane_b = T.row_b * (uint32_t)ny;
+    T.zb = 4u * (uint32_t)cell_index(T.z0 - 1 + lane, nz, open_bc);
+    T.zbo = 4u * (uint32_t)(lane > 0 ? lane - 1 : 0);
+    T.zout = lane >= 1 && lane <= nzo;
+#pragma unroll
+    for (int j = 0; j < TX + 2; ++j)
+        T.xoff[j] = (uint32_t)__builtin_amdgcn_readfirstlane(cell_index(T.x0 - 1 + j, nx, open_bc)) * T.plane_b;
+
+    // The loaded row: each column is reloaded with the next row right after
+    // its last use, so a full row of work hides every load's latency.
+    float cur[TX + 2][5];
+    {
+        const uint32_t yoff = (uint32_t)__builtin_amdgcn_readfirstlane(cell_index(T.y0 - 1, ny, open_bc)) * T.row_b;
+#pragma unroll
+        for (int j = 0; j < TX + 2; ++j) load_cell(T, T.xoff[j] + yoff, cur[j]);
+    }
+    // Y-stream state of the previous row, in two alternating sets so that
+    // rows hand it over without register copies.
+    QS qA[TX], qB[TX];
+    float fA[TX][5], fB[TX][5];
+    uint32_t vmax = 0;
+    int r = 0;
+    for (; r + 1 < T.nrows; r += 2) {
+        stream_row<TX>(T, g, r, cur, qA, fA, qB, fB, vmax);
+        stream_row<TX>(T, g, r + 1, cur, qB, fB, qA, fA, vmax);
+    }
+    if (r < T.nrows) stream_row<TX>(T, g, r, cur, qA, fA, qB, fB, vmax);
     return vmax;
 }
 
+#ifndef TF_TILE_ORDER
+#define TF_TILE_ORDER 0
+#endif
+// Tile order of consecutive waves: z-tiles fastest (0) or x-tiles fastest (1),
+// then the other, then y-chunks.
 __device__ __forceinline__ void decode_tile(int local, int ntz, int ntx, int &xt, int &zt, int &yc) {
+#if TF_TILE_ORDER == 0
     zt = local % ntz;
     const int rest = local / ntz;
     xt = rest % ntx;
     yc = rest / ntx;
+#else
+    xt = local % ntx;
+    const int rest = local / ntx;
+    zt = rest % ntz;
+    yc = rest / ntz;
+#endif
+}
+
+// Wave index with an XCD-aware block order. Workgroups are dispatched to the
+// 8 XCDs round-robin (block b runs on XCD b % 8), each with its own L2; this
+// gives XCD k the contiguous block range [k*Q, (k+1)*Q), so neighbouring
+// tiles, which share halo rows and cache lines, meet in the same L2.
+// Returns -1 for the padding blocks of a grid rounded up to 8*Q.
+__device__ __forceinline__ int xcd_wave_index(int total_waves) {
+    constexpr int kXcd = 8, kWavesPerBlock = kBlock / kWave;
+    const int nblocks = (total_waves + kWavesPerBlock - 1) / kWavesPerBlock;
+    const int q = (nblocks + kXcd - 1) / kXcd;
+    const int b = (int)blockIdx.x;
+    const int lb = (b % kXcd) * q + b / kXcd;
+    if (lb >= nblocks) return -1;
+    const int w = lb * kWavesPerBlock + wave_id_uniform();
+    return w < total_waves ? w : -1;
+}
+
+static inline unsigned xcd_grid(int total_waves) {
+    const int nblocks = (total_waves + (kBlock / kWave) - 1) / (kBlock / kWave);
+    return (unsigned)(((nblocks + 7) / 8) * 8);
 }
 
 // Fused step of one grid with an explicit dt (the _evolve_state entry point).
+// Waves per SIMD the register allocation must allow: the stream state grows
+// with TX, and below two resident waves nothing hides the load latency.
 template <int TX>
-__global__ __launch_bounds__(kBlock) void k_step_direct(SimDev S, const float *__restrict__ in,
+constexpr int waves_per_simd() { return TX <= 2 ? 4 : (TX <= 4 ? 3 : (TX <= 6 ? 2 : 1)); }
+
+template <int TX>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_per_simd<TX>()))) void k_step_direct(SimDev S, const float *__restrict__ in,
                                                         float *__restrict__ out, const double *__restrict__ dt,
                                                         double dx, uint32_t *__restrict__ bits_out) {
-    const int w = blockIdx.x * (kBlock / kWave) + wave_id_uniform();
-    if (w >= S.waves) return;
+    const int w = xcd_wave_index(S.waves);
+    if (w < 0) return;
     Gas g{S.gamma, S.gm1, S.inv_gm1, (float)(*dt / dx)};
     int xt, zt, yc;
     decode_tile(w, S.ntz, S.ntx, xt, zt, yc);
@@ -185,10 +306,10 @@ __global__ __launch_bounds__(kBlock) void k_step_direct(SimDev S, const float *_
 // Fused step of every active simulation of a table; simulation s owns waves
 // [prefix[s], prefix[s+1]). Input/output buffers follow the step parity.
 template <int TX>
-__global__ __launch_bounds__(kBlock) void k_step_sims(const SimDev *__restrict__ sims, const int *__restrict__ prefix,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_per_simd<TX>()))) void k_step_sims(const SimDev *__restrict__ sims, const int *__restrict__ prefix,
                                                       int nsims) {
-    const int w = blockIdx.x * (kBlock / kWave) + wave_id_uniform();
-    if (w >= prefix[nsims]) return;
+    const int w = xcd_wave_index(prefix[nsims]);
+    if (w < 0) return;
     int lo = 0, hi = nsims;
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
@@ -420,10 +541,11 @@ static inline dim3 grid1(int64_t n, int block = 256) { return dim3((unsigned)((n
 
 hipError_t launch_step_direct(int tx, const SimDev &S, const float *in, float *out, const double *dt, double dx,
                               uint32_t *bits, hipStream_t st) {
-    const dim3 grid((unsigned)((S.waves + 3) / 4)), block(kBlock);
+    const dim3 grid(xcd_grid(S.waves)), block(kBlock);
     switch (tx) {
         case 2: hipLaunchKernelGGL(k_step_direct<2>, grid, block, 0, st, S, in, out, dt, dx, bits); break;
         case 4: hipLaunchKernelGGL(k_step_direct<4>, grid, block, 0, st, S, in, out, dt, dx, bits); break;
+        case 6: hipLaunchKernelGGL(k_step_direct<6>, grid, block, 0, st, S, in, out, dt, dx, bits); break;
         case 8: hipLaunchKernelGGL(k_step_direct<8>, grid, block, 0, st, S, in, out, dt, dx, bits); break;
         default: return hipErrorInvalidValue;
     }
@@ -433,10 +555,11 @@ hipError_t launch_step_direct(int tx, const SimDev &S, const float *in, float *o
 hipError_t launch_step_sims(int tx, const SimDev *sims, const int *prefix, int nsims, int total_waves,
                             hipStream_t st) {
     if (total_waves <= 0) return hipSuccess;
-    const dim3 grid((unsigned)((total_waves + 3) / 4)), block(kBlock);
+    const dim3 grid(xcd_grid(total_waves)), block(kBlock);
     switch (tx) {
         case 2: hipLaunchKernelGGL(k_step_sims<2>, grid, block, 0, st, sims, prefix, nsims); break;
         case 4: hipLaunchKernelGGL(k_step_sims<4>, grid, block, 0, st, sims, prefix, nsims); break;
+        case 6: hipLaunchKernelGGL(k_step_sims<6>, grid, block, 0, st, sims, prefix, nsims); break;
         case 8: hipLaunchKernelGGL(k_step_sims<8>, grid, block, 0, st, sims, prefix, nsims); break;
         default: return hipErrorInvalidValue;
     }
