@@ -37,6 +37,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--tile", type=str, default="", help="force tile_x,tile_y (skips autotuning)")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=40)
+    ap.add_argument("--serve-rates", type=str, default="8,32", help="Poisson arrival rates (jobs/s) to sweep")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also copy the state buffer once (a plain copy kernel of known size, for HBM counters)")
     return ap.parse_args()
@@ -73,6 +77,72 @@ def cpu_baseline(nx, ny, budget_s):
                   f"z-slab, {steps} steps in {el:.1f} s, OpenMP over all host threads; "
                   f"host {os.cpu_count()} logical CPUs",
     }
+
+
+def measured_traffic(nx, ny, nz, tile):
+    """HBM bytes per launch of the fused step from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (scaled by a plain copy's counter reading),
+    when it was taken on this workload and tiling; else None."""
+    path = os.path.join(ROOT, "profiles", "fused_step_kh1024_hbm_traffic.md")
+    if (nx, ny, nz) != (1024, 1024, 1024) or not os.path.exists(path):
+        return None, None
+    fields = {}
+    for line in open(path):
+        if ":" in line and line.split(":")[0] in ("traffic_bytes_per_launch", "tile"):
+            k, v = line.split(":", 1)
+            fields[k.strip()] = v.strip()
+    if fields.get("tile") != f"{tile[0]},{tile[1]}":
+        return None, None
+    return int(fields["traffic_bytes_per_launch"]), os.path.relpath(path, ROOT)
+
+
+def batched_leg(dev):
+    """256 uneven blasts, sides {32,48,64,96,128}^3, 100 steps each: all
+    advanced together by one captured HIP graph, against the plain path that
+    runs them one after another through run_steps."""
+    import torch
+
+    from tinyfluids_amd import engine, ics
+
+    specs = ics.uneven_batch(256, 100, seed=3)
+    init = engine.upload(specs, dev)
+    engine.run_batched(specs[:16], init[:16], keep_states=False)  # warm-up
+    engine.run_plain(specs[:4], init[:4], keep_states=False)
+    b = engine.run_batched(specs, init, graph=True)
+    p = engine.run_plain(specs, init)
+    identical = all(torch.equal(x, y) for x, y in zip(b.states, p.states))
+    cells = sum(s.cells * s.steps for s in specs)
+    del init
+    return {
+        "workload": "256 seeded blasts, sides {32,48,64,96,128}^3, 100 steps each, gamma in {1.4,5/3}",
+        "sims": len(specs), "batched_s": b.seconds, "plain_s": p.seconds,
+        "sims_per_s": len(specs) / b.seconds, "plain_sims_per_s": len(specs) / p.seconds,
+        "speedup": p.seconds / b.seconds, "cell_updates_per_s": cells / b.seconds,
+        "plain_cell_updates_per_s": cells / p.seconds, "bitwise_identical_to_plain": identical,
+        "launch": "one HIP graph of 100 (fused multi-simulation step + per-simulation dt) pairs",
+    }
+
+
+def serve_leg(dev, njobs, rates):
+    """Poisson job stream (grids 32..256 per side, aspect up to 4:1, 50-2000
+    steps, blast or shear layer): continuous batching against first come,
+    first served one job at a time."""
+    from tinyfluids_amd import engine, ics
+
+    out = []
+    for rate in rates:
+        specs = ics.serving_stream(njobs, rate, seed=4)
+        srv = engine.Server(hbm_budget_bytes=64 << 30, steps_per_tick=16)
+        r = srv.serve(specs).summary()
+        p = engine.serve_plain(specs).summary()
+        out.append({"rate_jobs_per_s": rate, "jobs": njobs,
+                    "sims_per_s": r["sims_per_s"], "p50_latency_s": r["p50_latency_s"],
+                    "p99_latency_s": r["p99_latency_s"], "max_resident": r["max_resident"],
+                    "plain_sims_per_s": p["sims_per_s"], "plain_p50_latency_s": p["p50_latency_s"],
+                    "plain_p99_latency_s": p["p99_latency_s"],
+                    "speedup_sims_per_s": r["sims_per_s"] / p["sims_per_s"]})
+        print(f"bench: serve rate {rate}: {out[-1]}", file=sys.stderr, flush=True)
+    return out
 
 
 def main():
@@ -134,6 +204,7 @@ def main():
 
     cells = nx * ny * nz
     value = cells * args.steps / el
+    traffic, traffic_src = measured_traffic(nx, ny, nz, (tx.value, ly.value))
     kernel_s = stencil_ms / 1e3 / args.steps
     achieved = BYTES_PER_CELL * cells / kernel_s / 1e9
     rec = {
@@ -153,13 +224,20 @@ def main():
                                f"CFL 0.4, fused split HLL/Godunov step with dt",
                    "grid": [nx, ny, nz], "tile_x": tx.value, "tile_y": ly.value, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": f"tf::k_step_sims<{tx.value}>", "kernel_ms": kernel_s * 1e3,
                      "algorithmic_bytes_per_launch": BYTES_PER_CELL * cells},
         "sim_time": st.t,
     }
     del buf0, buf1, sim, batch
     torch.cuda.empty_cache()
+    if not args.no_batched:
+        rec["batched"] = batched_leg(dev)
+        print(f"bench: batched {rec['batched']}", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
+    if not args.no_serve:
+        rec["serving"] = serve_leg(dev, args.serve_jobs, [float(r) for r in args.serve_rates.split(",")])
+        torch.cuda.empty_cache()
     if not args.no_cpu:
         rec["cpu_baseline"] = cpu_baseline(nx, ny, args.cpu_seconds)
     print(json.dumps(rec), flush=True)

@@ -1,0 +1,111 @@
+"""GPU tests of the batched engine and the continuous-batching server:
+a simulation's result is bit-identical alone (plain single-grid loop),
+batched, graph-replayed, or served in a mix; batched results meet the
+float64 oracle; the server honours its HBM budget."""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import c_oracle, np_tinyfluids as npo
+from tinyfluids_amd import ics
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda:0")
+
+
+def specs_mixed():
+    return [
+        ics.SimSpec((16, 16, 16), ics.GAMMA_BLAST, 1 / 16, 7, "blast", (0.4, 0.5, 0.6), 0.2, 30.0),
+        ics.SimSpec((24, 20, 70), ics.GAMMA_KH, 1 / 20, 9, "kh"),
+        ics.SimSpec((32, 32, 32), ics.GAMMA_KH, 1 / 32, 5, "blast", (0.3, 0.3, 0.7), 0.12, 5.0),
+        ics.SimSpec((8, 40, 12), ics.GAMMA_BLAST, 1 / 40, 11, "blast", (0.5, 0.5, 0.5), 0.3, 50.0),
+        ics.SimSpec((10, 12, 130), ics.GAMMA_BLAST, 1 / 130, 6, "blast", (0.5, 0.5, 0.5), 0.2, 3.0),
+        ics.SimSpec((3, 5, 64), ics.GAMMA_KH, 1 / 64, 8, "kh"),
+    ]
+
+
+def test_batched_equals_solo_bitwise():
+    from tinyfluids_amd import engine
+    specs = specs_mixed()
+    init = engine.upload(specs, DEV)
+    b = engine.run_batched(specs, init, graph=True)
+    p = engine.run_plain(specs, init)
+    for i, (x, y, st) in enumerate(zip(b.states, p.states, b.statuses)):
+        assert st.n_iter == specs[i].steps and not st.active
+        assert torch.equal(x, y), f"sim {i} {specs[i].shape}"
+
+
+@pytest.mark.parametrize("tile_x,tile_y", [(2, 0), (4, 3), (8, 16), (6, 0)])
+def test_batched_tilings_bitwise(tile_x, tile_y):
+    from tinyfluids_amd import engine
+    specs = specs_mixed()
+    init = engine.upload(specs, DEV)
+    ref = engine.run_batched(specs, init, graph=False)
+    b = engine.run_batched(specs, init, graph=True, tile_x=tile_x, tile_y=tile_y)
+    for i, (x, y) in enumerate(zip(b.states, ref.states)):
+        assert torch.equal(x, y), (i, tile_x, tile_y)
+
+
+def test_graph_replays_match_eager():
+    """The captured graph is replayed several times, with eager work and
+    synchronisation in between; every replay matches eager stepping."""
+    from tinyfluids_amd.batch import Batch, Simulation
+    specs = specs_mixed()[:3]
+    a = [Simulation(s.initial_state(), s.dx, s.gamma, device=DEV) for s in specs]
+    b = [Simulation(s.initial_state(), s.dx, s.gamma, device=DEV) for s in specs]
+    ba, bb = Batch(a), Batch(b)
+    for s in a + b:
+        s.reset()
+    for rep in range(3):
+        ba.step_graph(4)
+        bb.step(4)
+        torch.cuda.synchronize()
+        _ = torch.ones(1000, device=DEV).sum().item()  # eager work between replays
+        sa, sb = ba.status(), bb.status()
+        for i in range(len(specs)):
+            assert sa[i].n_iter == sb[i].n_iter == 4 * (rep + 1)
+            assert sa[i].t == sb[i].t
+            assert torch.equal(a[i].state(sa[i]), b[i].state(sb[i])), (rep, i)
+
+
+def test_batched_meets_oracle():
+    from tinyfluids_amd import engine
+    spec = ics.SimSpec((40, 36, 44), ics.GAMMA_BLAST, 1 / 40, 20, "blast", (0.45, 0.5, 0.55), 0.15, 20.0)
+    other = specs_mixed()[:2]
+    specs = [spec] + other
+    init = engine.upload(specs, DEV)
+    res = engine.run_batched(specs, init)
+    ref, t_ref, _ = c_oracle.run_steps(spec.initial_state().astype(np.float64), spec.dx, spec.gamma, spec.steps)
+    err = npo.relative_l1(res.states[0].cpu().numpy(), ref, spec.gamma)
+    assert np.all(err < 1e-5), err
+    assert abs(res.statuses[0].t - t_ref) < 1e-6 * t_ref
+
+
+def test_server_results_same_alone_and_in_mix():
+    from tinyfluids_amd import engine
+    specs = specs_mixed()
+    for k, s in enumerate(specs):
+        s.arrival = 0.0 if k < 3 else 0.002 * k  # three at once, then a trickle
+    srv = engine.Server(hbm_budget_bytes=1 << 30, steps_per_tick=3, keep_states=True)
+    rep = srv.serve(specs)
+    alone = engine.serve_plain(specs, keep_states=True)
+    assert rep.max_resident >= 2  # jobs really overlapped
+    for a, b in zip(rep.jobs, alone.jobs):
+        assert a.steps_done == a.spec.steps
+        assert torch.equal(a.state, b.state), a.index
+        assert a.finished >= a.arrival
+
+
+def test_server_respects_hbm_budget():
+    from tinyfluids_amd import engine
+    specs = [ics.SimSpec((32, 32, 32), ics.GAMMA_KH, 1 / 32, 6, "kh", arrival=0.0) for _ in range(5)]
+    budget = 2 * engine.state_bytes(specs[0]) + 100
+    rep = engine.Server(hbm_budget_bytes=budget, steps_per_tick=2).serve(specs)
+    assert rep.max_resident <= 2
+    assert rep.peak_bytes <= budget
+    assert all(j.steps_done == 6 for j in rep.jobs)
+    s = rep.summary()
+    assert s["p99_latency_s"] >= s["p50_latency_s"] > 0

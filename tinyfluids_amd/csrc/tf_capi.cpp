@@ -369,8 +369,9 @@ struct tf_batch {
     SimDev *d_sims = nullptr;
     int *d_prefix = nullptr;
     Ctrl *d_status = nullptr;
-    std::vector<SimDev> h_sims;
+    std::vector<SimDev> h_sims;  // device table order
     std::vector<int> h_prefix;
+    std::vector<int> order;      // h_sims[k] is the caller's simulation order[k]
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipStream_t capture_stream = nullptr;
@@ -406,59 +407,63 @@ int tf_batch_create(tf_batch **out, const tf_sim_desc *sims, int nsims, void *wo
     TF_REQUIRE(sims && workspace, "null pointer");
     TF_REQUIRE(workspace_bytes >= tf_batch_workspace_bytes(nsims), "workspace too small: %zu < %zu",
                workspace_bytes, tf_batch_workspace_bytes(nsims));
-    const int tx = tile_x == 0 ? 4 : tile_x;
+    const int tx = tile_x == 0 ? 8 : tile_x;
     TF_REQUIRE(valid_tx(tx), "tile_x must be 0, 2, 4, 6 or 8 (got %d)", tile_x);
     TF_REQUIRE(tile_y >= 0, "tile_y must be >= 0");
+    for (int i = 0; i < nsims; ++i) {
+        const tf_sim_desc &d = sims[i];
+        if (int e = check_dims(d.nx, d.ny, d.nz)) return e;
+        if (int e = check_fused_size(d.nx, d.ny, d.nz)) return e;
+        if (int e = check_open_bc(d.handle_boundaries, d.nx, d.ny, d.nz)) return e;
+        TF_REQUIRE(d.buf0 && d.buf1 && d.ctrl, "null device pointer in sim %d", i);
+    }
+    auto build = [&](int ly_cap, std::vector<SimDev> &tab) {
+        long total = 0;
+        for (int i = 0; i < nsims; ++i) {
+            const tf_sim_desc &d = sims[i];
+            const int ly = tile_y > 0 ? tile_y : std::max(1, std::min(d.ny, ly_cap));
+            SimDev S = make_simdev(d.nx, d.ny, d.nz, d.handle_boundaries, d.gamma, tx, ly);
+            S.buf[0] = d.buf0;
+            S.buf[1] = d.buf1;
+            S.ctrl = reinterpret_cast<Ctrl *>(d.ctrl);
+            tab[i] = S;
+            total += S.waves;
+        }
+        return total;
+    };
+    std::vector<SimDev> tab(nsims);
+    long total = build(1 << 30, tab);
+    if (tile_y == 0) {
+        // Whole y-lines per wave unless the batch is too small to fill the
+        // chip several times over; then halve every simulation's y-chunks
+        // (down to 16 rows).
+        const long target = 256L * 4 * 3 * 6;
+        for (int cap = 1 << 30; total < target && cap > 16;) {
+            int maxny = 1;
+            for (int i = 0; i < nsims; ++i) maxny = std::max(maxny, sims[i].ny);
+            cap = std::min(cap, maxny) / 2;
+            if (cap < 16) cap = 16;
+            total = build(cap, tab);
+        }
+    }
+    TF_REQUIRE(total <= 2000000000L, "batch too large");
     auto *b = new tf_batch();
     b->nsims = nsims;
     b->tx = tx;
+    // Longest waves first (rows per wave), so that the short ones fill the
+    // tail of every launch.
+    b->order.resize(nsims);
+    for (int i = 0; i < nsims; ++i) b->order[i] = i;
+    std::stable_sort(b->order.begin(), b->order.end(), [&](int a, int c) {
+        const long la = (long)std::min(tab[a].ly, tab[a].ny), lc = (long)std::min(tab[c].ly, tab[c].ny);
+        return la > lc;
+    });
     b->h_sims.resize(nsims);
     b->h_prefix.resize(nsims + 1);
     b->h_prefix[0] = 0;
-    for (int i = 0; i < nsims; ++i) {
-        const tf_sim_desc &d = sims[i];
-        int rc = check_dims(d.nx, d.ny, d.nz);
-        if (rc == TF_OK) rc = check_fused_size(d.nx, d.ny, d.nz);
-        if (rc == TF_OK) rc = check_open_bc(d.handle_boundaries, d.nx, d.ny, d.nz);
-        if (rc == TF_OK && !(d.buf0 && d.buf1 && d.ctrl)) rc = fail(TF_ERR_ARG, "null device pointer in sim %d", i);
-        if (rc != TF_OK) {
-            delete b;
-            return rc;
-        }
-        const int ly = tile_y > 0 ? tile_y : d.ny;  // tile_y == 0: refined below
-        SimDev S = make_simdev(d.nx, d.ny, d.nz, d.handle_boundaries, d.gamma, tx, ly);
-        S.buf[0] = d.buf0;
-        S.buf[1] = d.buf1;
-        S.ctrl = reinterpret_cast<Ctrl *>(d.ctrl);
-        b->h_sims[i] = S;
-        const long next = (long)b->h_prefix[i] + S.waves;
-        if (next > 2000000000L) {
-            delete b;
-            return fail(TF_ERR_ARG, "batch too large");
-        }
-        b->h_prefix[i + 1] = (int)next;
-    }
-    if (tile_y == 0) {
-        // Whole y-lines per wave unless the batch is too small to fill the
-        // chip; then halve the y-chunks of every simulation (down to 16 rows).
-        const long target = 256L * 4 * 3 * 6;
-        for (int ly_cap = 1 << 30; ; ly_cap /= 2) {
-            long total = 0;
-            for (int i = 0; i < nsims; ++i) {
-                const tf_sim_desc &d = sims[i];
-                const int ly = std::max(1, std::min(d.ny, std::max(16, ly_cap)));
-                b->h_sims[i] = [&] {
-                    SimDev S = make_simdev(d.nx, d.ny, d.nz, d.handle_boundaries, d.gamma, tx, ly);
-                    S.buf[0] = d.buf0;
-                    S.buf[1] = d.buf1;
-                    S.ctrl = reinterpret_cast<Ctrl *>(d.ctrl);
-                    return S;
-                }();
-                total += b->h_sims[i].waves;
-                b->h_prefix[i + 1] = (int)std::min(total, 2000000000L);
-            }
-            if (total >= target || ly_cap <= 16) break;
-        }
+    for (int k = 0; k < nsims; ++k) {
+        b->h_sims[k] = tab[b->order[k]];
+        b->h_prefix[k + 1] = b->h_prefix[k] + b->h_sims[k].waves;
     }
     b->total_waves = b->h_prefix[nsims];
     char *p = static_cast<char *>(workspace);
@@ -575,7 +580,7 @@ int tf_batch_status(tf_batch *b, tf_sim_status *host_out, void *stream) {
     std::vector<Ctrl> tmp(b->nsims);
     TF_HIP(hipMemcpyAsync(tmp.data(), b->d_status, b->nsims * sizeof(Ctrl), hipMemcpyDeviceToHost, st));
     TF_HIP(hipStreamSynchronize(st));
-    for (int i = 0; i < b->nsims; ++i) std::memcpy(&host_out[i], &tmp[i], sizeof(tf_sim_status));
+    for (int k = 0; k < b->nsims; ++k) std::memcpy(&host_out[b->order[k]], &tmp[k], sizeof(tf_sim_status));
     return TF_OK;
 }
 

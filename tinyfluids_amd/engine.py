@@ -1,0 +1,237 @@
+"""Multi-simulation workloads on one GPU: the uneven batch and the served
+job stream, each next to the plain path that runs one grid per launch
+sequence (the reference's own usage: one time_integration call per grid).
+
+- ``run_batched``: every simulation resident in HBM, all advanced together,
+  ``steps`` batched steps captured once into a HIP graph and replayed as one
+  launch.
+- ``run_plain``: the same simulations one after another through
+  ``run_steps`` (the drop-in single-grid loop).
+- ``Server``: continuous batching. Jobs arrive over time; a job is admitted
+  when its two state buffers fit the HBM budget and a slot is free; every
+  tick advances all admitted jobs by a few batched steps; finished jobs leave
+  and free their memory; newly admitted jobs join the next tick while the
+  others keep stepping. ``serve_plain`` serves the same stream first come,
+  first served, one job at a time.
+
+Per-cell arithmetic does not depend on what else is in a launch, and the CFL
+maximum is an exact max, so a simulation's result is bit-identical alone,
+batched, or served in any mix.
+"""
+
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib
+from .batch import Batch, Simulation
+from .ics import SimSpec
+from .time_integration import run_steps
+
+STATE_BYTES_PER_CELL = 2 * 5 * 4  # two float32 state buffers
+
+
+def state_bytes(spec: SimSpec) -> int:
+    return STATE_BYTES_PER_CELL * spec.cells + _lib.TF_CTRL_BYTES
+
+
+def upload(specs: Sequence[SimSpec], device) -> List[torch.Tensor]:
+    return [torch.from_numpy(s.initial_state()).to(device) for s in specs]
+
+
+@dataclass
+class BatchResult:
+    seconds: float
+    states: List[torch.Tensor]
+    statuses: list
+    waves: int = 0
+
+
+def run_batched(specs: Sequence[SimSpec], initial: Sequence[torch.Tensor], *, graph: bool = True, tile_x: int = 0,
+                tile_y: int = 0, keep_states: bool = True) -> BatchResult:
+    """All simulations advanced together for max(spec.steps) batched steps
+    (each stops at its own step count). Timed from reset to completion."""
+    device = initial[0].device
+    sims = [Simulation(x, s.dx, s.gamma, max_steps=s.steps, device=device) for s, x in zip(specs, initial)]
+    batch = Batch(sims, tile_x=tile_x, tile_y=tile_y)
+    steps = max(s.steps for s in specs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in sims:
+        s.reset()
+    if graph:
+        batch.step_graph(steps)
+    else:
+        batch.step(steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    st = batch.status()
+    states = [s.state(x) for s, x in zip(sims, st)] if keep_states else []
+    batch.close()
+    return BatchResult(el, states, st)
+
+
+def run_plain(specs: Sequence[SimSpec], initial: Sequence[torch.Tensor], keep_states: bool = True) -> BatchResult:
+    """One grid at a time through the single-grid loop."""
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = []
+    for s, x in zip(specs, initial):
+        state, _, _ = run_steps(x, s.dx, s.gamma, s.steps)
+        if keep_states:
+            out.append(state)
+    torch.cuda.synchronize()
+    return BatchResult(time.perf_counter() - t0, out, [])
+
+
+# ------------------------------------------------------------------ serving
+@dataclass
+class JobRecord:
+    index: int
+    spec: SimSpec
+    arrival: float
+    admitted: float = math.nan
+    finished: float = math.nan
+    steps_done: int = 0
+    nonfinite: bool = False
+    state: Optional[torch.Tensor] = None
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+@dataclass
+class ServeReport:
+    jobs: List[JobRecord]
+    makespan: float
+    ticks: int = 0
+    max_resident: int = 0
+    peak_bytes: int = 0
+    extra: Dict[str, float] = field(default_factory=dict)
+
+    def summary(self) -> Dict[str, float]:
+        lat = np.array([j.latency for j in self.jobs])
+        return {
+            "jobs": len(self.jobs),
+            "sims_per_s": len(self.jobs) / self.makespan,
+            "p50_latency_s": float(np.percentile(lat, 50)),
+            "p99_latency_s": float(np.percentile(lat, 99)),
+            "mean_latency_s": float(lat.mean()),
+            "makespan_s": self.makespan,
+            "cell_updates_per_s": sum(j.spec.cells * j.spec.steps for j in self.jobs) / self.makespan,
+            "ticks": self.ticks,
+            "max_resident": self.max_resident,
+            "peak_state_bytes": self.peak_bytes,
+        }
+
+
+class Server:
+    """Continuous-batching server of simulation jobs on one GPU."""
+
+    def __init__(self, hbm_budget_bytes: int, max_resident: int = 1024, steps_per_tick: int = 8, device=None,
+                 keep_states: bool = False, tile_x: int = 0):
+        self.budget = int(hbm_budget_bytes)
+        self.max_resident = int(max_resident)
+        self.steps_per_tick = int(steps_per_tick)
+        self.device = device or torch.device("cuda:0")
+        self.keep_states = keep_states
+        self.tile_x = tile_x
+
+    def serve(self, specs: Sequence[SimSpec], time_scale: float = 1.0) -> ServeReport:
+        """Serves jobs arriving at spec.arrival * time_scale seconds after the
+        start (wall clock). Returns per-job records."""
+        jobs = [JobRecord(i, s, s.arrival * time_scale) for i, s in enumerate(specs)]
+        for j in jobs:
+            if state_bytes(j.spec) > self.budget:
+                raise ValueError(f"job {j.index} ({j.spec.shape}) does not fit the HBM budget")
+        pending = list(jobs)  # in arrival order
+        queue: List[JobRecord] = []
+        active: Dict[int, Simulation] = {}
+        used = 0
+        peak = 0
+        ticks = 0
+        max_res = 0
+        batch: Optional[Batch] = None
+        dirty = True
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        done = 0
+        while done < len(jobs):
+            now = time.perf_counter() - t0
+            while pending and pending[0].arrival <= now:
+                queue.append(pending.pop(0))
+            # admission in arrival order while memory and slots allow
+            while queue and len(active) < self.max_resident and used + state_bytes(queue[0].spec) <= self.budget:
+                j = queue.pop(0)
+                s = j.spec
+                sim = Simulation(s.initial_state(), s.dx, s.gamma, max_steps=s.steps, device=self.device)
+                sim.reset()
+                active[j.index] = sim
+                used += state_bytes(s)
+                j.admitted = time.perf_counter() - t0
+                dirty = True
+            peak = max(peak, used)
+            max_res = max(max_res, len(active))
+            if not active:
+                # idle: wait for the next arrival
+                if pending:
+                    time.sleep(max(0.0, min(0.01, pending[0].arrival - (time.perf_counter() - t0))))
+                continue
+            if dirty:
+                if batch is not None:
+                    batch.close()
+                ids = list(active)
+                batch = Batch([active[i] for i in ids], tile_x=self.tile_x)
+                dirty = False
+            batch.step(self.steps_per_tick)
+            ticks += 1
+            statuses = batch.status()  # waits for the tick
+            now = time.perf_counter() - t0
+            for i, st in zip(ids, statuses):
+                if not st.active:
+                    j = jobs[i]
+                    sim = active.pop(i)
+                    j.finished = now
+                    j.steps_done = int(st.n_iter)
+                    j.nonfinite = bool(st.nonfinite)
+                    if self.keep_states:
+                        j.state = sim.state(st).clone()
+                    used -= state_bytes(j.spec)
+                    done += 1
+                    dirty = True
+        if batch is not None:
+            batch.close()
+        makespan = time.perf_counter() - t0
+        return ServeReport(jobs, makespan, ticks, max_res, peak)
+
+
+def serve_plain(specs: Sequence[SimSpec], time_scale: float = 1.0, device=None,
+                keep_states: bool = False) -> ServeReport:
+    """First come, first served, one job at a time through run_steps."""
+    device = device or torch.device("cuda:0")
+    jobs = [JobRecord(i, s, s.arrival * time_scale) for i, s in enumerate(specs)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    peak = 0
+    for j in jobs:
+        now = time.perf_counter() - t0
+        if j.arrival > now:
+            time.sleep(j.arrival - now)
+        j.admitted = time.perf_counter() - t0
+        x = torch.from_numpy(j.spec.initial_state()).to(device)
+        state, _, nonfinite = run_steps(x, j.spec.dx, j.spec.gamma, j.spec.steps)
+        torch.cuda.synchronize()
+        j.finished = time.perf_counter() - t0
+        j.steps_done = j.spec.steps
+        j.nonfinite = nonfinite
+        if keep_states:
+            j.state = state
+        peak = max(peak, state_bytes(j.spec))
+    return ServeReport(jobs, time.perf_counter() - t0, len(jobs), 1, peak)
