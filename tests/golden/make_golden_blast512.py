@@ -1,0 +1,65 @@
+"""Writes tests/golden/blast512_noise.npz: the float64 oracle on the 512^3
+periodic blast with seeded density noise (rho = 1 + 0.01*U(-1,1) from
+default_rng(1); p = 10 inside r < 0.1 of the box centre, 1 outside; v = 0;
+gamma 5/3; CFL 0.4; dx = 1/512), 200 steps.
+
+The full state is 5.4 GB per float64 copy, so the fixture keeps
+- every 32nd cell along each axis (16^3 cells x 5 primitive fields) at steps
+  0, 50, 100, 150 and 200 (float64),
+- the float64 totals of mass, momentum and energy every 10 steps,
+- the float64 dt of every step.
+
+Runs the scalar C oracle (OpenMP) for about half an hour on 8 cores.
+Run from the repository root: python tests/golden/make_golden_blast512.py
+"""
+
+from __future__ import annotations
+
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from oracle import c_oracle  # noqa: E402
+from tinyfluids_amd import ics  # noqa: E402
+
+N = 512
+STEPS = 200
+SUB = 32
+CHECK = (0, 50, 100, 150, 200)
+
+
+def main():
+    s32, dx = ics.blast_noise(N, seed=1)
+    state = s32.astype(np.float64)
+    del s32
+    g = ics.GAMMA_BLAST
+    sub = {}
+    totals = []
+    dts = []
+    t0 = time.time()
+    for step in range(0, STEPS + 1, 10):
+        if step % 10 == 0:
+            totals.append(c_oracle.conserved_totals(state, g))
+        if step in CHECK:
+            sub[f"sub_{step}"] = state[:, ::SUB, ::SUB, ::SUB].copy()
+        if step == STEPS:
+            break
+        _, n, _, d = c_oracle.time_integration(state, dx, float("inf"), g, False, 10, 0.4, in_place=True,
+                                               record_dt=True)
+        assert n == 10
+        dts.extend(d.tolist())
+        print(f"step {step + 10}: {time.time() - t0:.0f} s", flush=True)
+    np.savez_compressed(os.path.join(HERE, "blast512_noise.npz"), totals=np.array(totals), dts=np.array(dts),
+                        stride=np.array(SUB), checkpoints=np.array(CHECK), dx=np.array(dx), gamma=np.array(g),
+                        **sub)
+    print("done", time.time() - t0)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,116 @@
+"""Parity at the benchmark sizes.
+
+Kelvin-Helmholtz (gamma 1.4, 50 steps) on 1024^3 and 1536x512x512: the
+initial state does not depend on z and the z-sweep of a z-uniform state is a
+round trip, so the float64 oracle on the nx x ny x 1 grid is the exact
+reduction of the 3-D oracle (bit-for-bit in the oracle). The GPU result must
+have identical z-planes, match the reduced oracle to relative L1 <= 1e-5 per
+conserved field, keep m_z exactly zero and conserve mass in float64 totals.
+
+512^3 blast with seeded density noise (gamma 5/3, 200 steps): compared with
+the float64 oracle fixture tests/golden/blast512_noise.npz (every 32nd cell
+at steps 0/50/100/150/200, float64 totals every 10 steps, every dt).
+"""
+
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import c_oracle, np_tinyfluids as npo
+from tinyfluids_amd import ics
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda:0")
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+TOL = 1e-5
+
+
+def run_resident(buf0, dx, gamma, steps, tile_x=0):
+    from tinyfluids_amd.batch import Batch, Simulation
+    buf1 = torch.empty_like(buf0)
+    sim = Simulation(buf0, dx, gamma, max_steps=steps, buffers=(buf0, buf1))
+    b = Batch([sim], tile_x=tile_x)
+    sim.reset()
+    b.step(steps)
+    st = b.status()[0]
+    assert st.n_iter == steps and not st.nonfinite
+    return sim.state(st), st
+
+
+@pytest.mark.parametrize("shape,strict", [((1024, 1024, 1024), True), ((1536, 512, 512), True),
+                                          ((8, 128, 1 << 20), False)])
+def test_kelvin_helmholtz_full_size(shape, strict):
+    """strict: relative L1 <= 1e-5 per conserved field (the benchmark grids).
+    Otherwise (a 2^30-cell grid that exercises the far-row kernel, with a
+    tiny m_y field): within max(1e-5, 2x the error of the reference algorithm
+    run in float32, the reference's own precision)."""
+    import tinyfluids_amd as tf
+    nx, ny, nz = shape
+    dx = 1.0 / ny
+    steps = 50
+    print(f"KH {shape}, {steps} steps", flush=True)
+    plane = ics.kelvin_helmholtz_plane(nx, ny, dx)
+    buf0 = torch.from_numpy(plane).to(DEV)[:, :, :, None].expand(5, nx, ny, nz).contiguous()
+    tot0 = tf.conserved_totals(buf0, ics.GAMMA_KH).cpu().numpy()
+    out, st = run_resident(buf0, dx, ics.GAMMA_KH, steps)
+    # z-invariance: every z-plane identical
+    assert torch.equal(out, out[:, :, :, :1].expand_as(out))
+    assert int(torch.count_nonzero(out[3])) == 0  # vz stays exactly 0
+    tot = tf.conserved_totals(out, ics.GAMMA_KH).cpu().numpy()
+    got = out[:, :, :, :1].cpu().numpy()
+    del out, buf0
+    torch.cuda.empty_cache()
+    ref, t_ref, _ = c_oracle.run_steps(plane[:, :, :, None].astype(np.float64), dx, ics.GAMMA_KH, steps)
+    err = npo.relative_l1(got, ref, ics.GAMMA_KH)
+    ref32, _, _ = c_oracle.run_steps(plane[:, :, :, None], dx, ics.GAMMA_KH, steps)
+    err32 = npo.relative_l1(ref32, ref, ics.GAMMA_KH)
+    print("KH rel L1 per conserved field", err, "reference float32", err32, "t", st.t, t_ref, flush=True)
+    bound = np.full(5, TOL) if strict else np.maximum(TOL, 2.0 * err32)
+    assert np.all(err[[0, 1, 2, 4]] < bound[[0, 1, 2, 4]]), (err, bound)
+    assert err[3] == 0.0
+    assert abs(st.t - t_ref) <= 1e-5 * t_ref
+    assert abs(tot[0] - tot0[0]) <= 1e-7 * tot0[0]
+    assert abs(tot[4] - tot0[4]) <= 1e-5 * tot0[4]
+
+
+def test_blast_noise_512_against_float64_fixture():
+    path = os.path.join(GOLDEN, "blast512_noise.npz")
+    if not os.path.exists(path):
+        pytest.skip("fixture tests/golden/blast512_noise.npz not generated")
+    import tinyfluids_amd as tf
+    from tinyfluids_amd.batch import Batch, Simulation
+    g = np.load(path, allow_pickle=False)
+    stride = int(g["stride"])
+    gamma = float(g["gamma"])
+    dx = float(g["dx"])
+    s0, dx0 = ics.blast_noise(512, seed=1)
+    assert dx0 == dx
+    buf0 = torch.from_numpy(s0).to(DEV)
+    del s0
+    buf1 = torch.empty_like(buf0)
+    sim = Simulation(buf0, dx, gamma, buffers=(buf0, buf1))
+    b = Batch([sim])
+    sim.reset()
+    totals = [tf.conserved_totals(buf0, gamma).cpu().numpy()]
+    errs = {}
+    for step in range(10, 201, 10):
+        b.step(10)
+        st = b.status()[0]
+        assert st.n_iter == step and not st.nonfinite
+        cur = sim.state(st)
+        totals.append(tf.conserved_totals(cur, gamma).cpu().numpy())
+        if step % 50 == 0:
+            sub = cur[:, ::stride, ::stride, ::stride].cpu().numpy()
+            errs[step] = npo.relative_l1(sub, g[f"sub_{step}"], gamma)
+    print("512^3 blast+noise subsample rel L1", errs, flush=True)
+    totals = np.array(totals)
+    ref_tot = g["totals"]
+    for step, e in errs.items():
+        assert np.all(e < TOL), (step, e)
+    # float64 totals: mass to float32 round-off, energy to float32 round-off accumulated over 600 sweeps
+    assert np.all(np.abs(totals[:, 0] - ref_tot[:, 0]) <= 1e-6 * ref_tot[0, 0])
+    assert np.all(np.abs(totals[:, 4] - ref_tot[:, 4]) <= 1e-5 * ref_tot[0, 4])
+    assert np.all(np.abs(totals[:, 1:4]) <= 1e-6 * ref_tot[0, 0])
+    assert abs(st.t - float(np.sum(g["dts"]))) <= 1e-5 * float(np.sum(g["dts"]))

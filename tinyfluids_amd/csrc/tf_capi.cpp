@@ -21,6 +21,10 @@ hipError_t launch_step_direct(int tx, const SimDev &S, const float *in, float *o
                               uint32_t *bits, hipStream_t st);
 hipError_t launch_step_sims(int tx, const SimDev *sims, const int *prefix, int nsims, int total_waves,
                             hipStream_t st);
+hipError_t launch_step_direct_far(int tx, const SimDev &S, const FarList &F, const float *in, float *out,
+                                  const double *dt, double dx, uint32_t *bits, hipStream_t st);
+hipError_t launch_step_sims_far(int tx, const SimDev *sims, const FarTile *far, int nfar, int max_ntz,
+                                hipStream_t st);
 hipError_t launch_finalize_sims(const SimDev *sims, int nsims, hipStream_t st);
 hipError_t launch_finalize_one(Ctrl *C, int first, hipStream_t st);
 hipError_t launch_ctrl_reset(Ctrl *C, double dx, double c_cfl, double t_final, int64_t max_steps, hipStream_t st);
@@ -133,7 +137,31 @@ SimDev make_simdev(int nx, int ny, int nz, int open_bc, double gamma, int tx, in
     S.ntz = (nz + kZT - 1) / kZT;
     S.nyc = (ny + S.ly - 1) / S.ly;
     S.waves = S.ntx * S.ntz * S.nyc;
+    // fields of exactly 4 GiB: the last dword is out of reach of a buffer resource
+    S.far = ((int64_t)nx * ny * nz == (int64_t)1 << 30) ? 1 : 0;
     return S;
+}
+
+// The tiles of S that touch the last (x, y) row (the device-side predicate
+// touches_last_row, evaluated for every tile); appended for simulation `sim`.
+void far_tiles(const SimDev &S, int tx, int sim, std::vector<FarTile> &out) {
+    if (!S.far) return;
+    for (int yc = 0; yc < S.nyc; ++yc)
+        for (int xt = 0; xt < S.ntx; ++xt) {
+            const bool xr = xt == 0 || xt * tx + tx >= S.nx - 1;
+            const bool yr = yc == 0 || yc * S.ly + S.ly >= S.ny - 1;
+            if (xr && yr) out.push_back(FarTile{sim, xt, yc, 0});
+        }
+}
+
+FarList far_list(const SimDev &S, int tx) {
+    std::vector<FarTile> v;
+    far_tiles(S, tx, 0, v);
+    FarList F;
+    std::memset(&F, 0, sizeof F);
+    F.n = (int)std::min<size_t>(v.size(), kMaxFarPerSim);
+    for (int i = 0; i < F.n; ++i) F.t[i] = v[i];
+    return F;
 }
 
 bool valid_tx(int tx) { return tx == 2 || tx == 4 || tx == 6 || tx == 8; }
@@ -226,6 +254,7 @@ int tf_evolve_state(const float *in, float *out, int nx, int ny, int nz, double 
     tiling_for(nx, ny, nz, tx, ly);
     const SimDev S = make_simdev(nx, ny, nz, handle_boundaries, gamma, tx, ly);
     TF_HIP(launch_step_direct(tx, S, in, out, dt, grid_spacing, max_speed_out, S_(stream)));
+    TF_HIP(launch_step_direct_far(tx, S, far_list(S, tx), in, out, dt, grid_spacing, max_speed_out, S_(stream)));
     return TF_OK;
 }
 
@@ -248,8 +277,10 @@ int tf_time_integration(float *state, float *scratch, int nx, int ny, int nz, do
     S.buf[1] = scratch;
     S.ctrl = &H->ctrl;
     const int prefix[2] = {0, S.waves};
+    const FarList F = far_list(S, tx);
     TF_HIP(hipMemcpyAsync(&H->sim, &S, sizeof S, hipMemcpyHostToDevice, st));
     TF_HIP(hipMemcpyAsync(H->prefix, prefix, sizeof prefix, hipMemcpyHostToDevice, st));
+    TF_HIP(hipMemcpyAsync(&H->far, &F, sizeof F, hipMemcpyHostToDevice, st));
     TF_HIP(launch_ctrl_reset(&H->ctrl, grid_spacing, c_cfl, t_final, max_steps, st));
     TF_HIP(launch_speed(state, (int64_t)nx * ny * nz, (float)gamma, &H->ctrl.max_bits[0], st));
     TF_HIP(launch_finalize_one(&H->ctrl, 1, st));
@@ -260,6 +291,7 @@ int tf_time_integration(float *state, float *scratch, int nx, int ny, int nz, do
     for (;;) {
         for (int i = 0; i < chunk; ++i) {
             TF_HIP(launch_step_sims(tx, &H->sim, H->prefix, 1, S.waves, st));
+            TF_HIP(launch_step_sims_far(tx, &H->sim, H->far.t, F.n, S.ntz, st));
             TF_HIP(launch_finalize_sims(&H->sim, 1, st));
         }
         TF_HIP(hipMemcpyAsync(&host, &H->ctrl, sizeof host, hipMemcpyDeviceToHost, st));
@@ -326,11 +358,15 @@ int tf_autotune_evolve(const float *in, float *out, int nx, int ny, int nz, doub
         if (tx > 2 && tx > nx) continue;
         for (int ly : lys) {
             const SimDev S = make_simdev(nx, ny, nz, 0, gamma, tx, ly);
+            const FarList F = far_list(S, tx);
             hipError_t e = launch_step_direct(tx, S, in, out, dt, grid_spacing, nullptr, st);
+            if (e == hipSuccess) e = launch_step_direct_far(tx, S, F, in, out, dt, grid_spacing, nullptr, st);
             const int reps = 3;
             if (e == hipSuccess) e = hipEventRecord(e0, st);
-            for (int r = 0; r < reps && e == hipSuccess; ++r)
+            for (int r = 0; r < reps && e == hipSuccess; ++r) {
                 e = launch_step_direct(tx, S, in, out, dt, grid_spacing, nullptr, st);
+                if (e == hipSuccess) e = launch_step_direct_far(tx, S, F, in, out, dt, grid_spacing, nullptr, st);
+            }
             if (e == hipSuccess) e = hipEventRecord(e1, st);
             if (e == hipSuccess) e = hipEventSynchronize(e1);
             float ms = 0.f;
@@ -372,6 +408,9 @@ struct tf_batch {
     std::vector<SimDev> h_sims;  // device table order
     std::vector<int> h_prefix;
     std::vector<int> order;      // h_sims[k] is the caller's simulation order[k]
+    FarTile *d_far = nullptr;    // far-row tiles (table indices), see far_tiles
+    int nfar = 0;
+    int far_max_ntz = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipStream_t capture_stream = nullptr;
@@ -383,7 +422,7 @@ static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 size_t tf_batch_workspace_bytes(int nsims) {
     if (nsims < 0) nsims = 0;
     return align256((size_t)nsims * sizeof(SimDev)) + align256((size_t)(nsims + 1) * sizeof(int)) +
-           align256((size_t)nsims * sizeof(Ctrl));
+           align256((size_t)nsims * sizeof(Ctrl)) + align256((size_t)nsims * kMaxFarPerSim * sizeof(FarTile));
 }
 
 int tf_sim_reset(const tf_sim_desc *sim, void *stream) {
@@ -472,10 +511,21 @@ int tf_batch_create(tf_batch **out, const tf_sim_desc *sims, int nsims, void *wo
     b->d_prefix = reinterpret_cast<int *>(p);
     p += align256((size_t)(nsims + 1) * sizeof(int));
     b->d_status = reinterpret_cast<Ctrl *>(p);
+    p += align256((size_t)nsims * sizeof(Ctrl));
+    b->d_far = reinterpret_cast<FarTile *>(p);
+    std::vector<FarTile> far;
+    for (int k = 0; k < nsims; ++k) {
+        const size_t before = far.size();
+        far_tiles(b->h_sims[k], tx, k, far);
+        if (far.size() > before) b->far_max_ntz = std::max(b->far_max_ntz, b->h_sims[k].ntz);
+    }
+    b->nfar = (int)far.size();
     hipStream_t st = S_(stream);
     hipError_t e = hipMemcpyAsync(b->d_sims, b->h_sims.data(), nsims * sizeof(SimDev), hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
         e = hipMemcpyAsync(b->d_prefix, b->h_prefix.data(), (nsims + 1) * sizeof(int), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && b->nfar > 0)
+        e = hipMemcpyAsync(b->d_far, far.data(), far.size() * sizeof(FarTile), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
         delete b;
@@ -497,6 +547,7 @@ int tf_batch_destroy(tf_batch *b) {
 static int enqueue_steps(tf_batch *b, int steps, hipStream_t st) {
     for (int i = 0; i < steps; ++i) {
         TF_HIP(launch_step_sims(b->tx, b->d_sims, b->d_prefix, b->nsims, b->total_waves, st));
+        TF_HIP(launch_step_sims_far(b->tx, b->d_sims, b->d_far, b->nfar, b->far_max_ntz, st));
         TF_HIP(launch_finalize_sims(b->d_sims, b->nsims, st));
     }
     return TF_OK;
@@ -553,6 +604,7 @@ int tf_batch_step_timed(tf_batch *b, int steps, float *stencil_ms, void *stream)
     for (int i = 0; i < steps && rc == TF_OK; ++i) {
         hipError_t e = hipEventRecord(ev[2 * i], st);
         if (e == hipSuccess) e = launch_step_sims(b->tx, b->d_sims, b->d_prefix, b->nsims, b->total_waves, st);
+        if (e == hipSuccess) e = launch_step_sims_far(b->tx, b->d_sims, b->d_far, b->nfar, b->far_max_ntz, st);
         if (e == hipSuccess) e = hipEventRecord(ev[2 * i + 1], st);
         if (e == hipSuccess) e = launch_finalize_sims(b->d_sims, b->nsims, st);
         if (e != hipSuccess) rc = fail(TF_ERR_HIP, "timed step: %s", hipGetErrorString(e));

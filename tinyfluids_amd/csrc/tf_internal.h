@@ -40,7 +40,20 @@ struct SimDev {
     float gamma, gm1, inv_gm1, pad;
     int ntx, ntz, nyc, ly;  // tiles along x (of TX), z (of kZT), y-chunks and their length
     int waves;              // ntx * ntz * nyc
-    int pad2[3];
+    int far;                // fields of exactly 4 GiB: tiles touching the last (x, y) row run in k_step_far
+    int pad2[2];
+};
+
+// A tile handled by the far-row kernel: simulation index, x-tile, y-chunk.
+struct FarTile {
+    int sim, xt, yc, pad;
+};
+constexpr int kMaxFarPerSim = 9;  // at most 3 x-tiles x 3 y-chunks touch the last row
+
+struct FarList {
+    int n;
+    int pad;
+    FarTile t[kMaxFarPerSim];
 };
 
 // Workspace header of the single-grid entry points.
@@ -50,6 +63,7 @@ struct WsHeader {
     int prefix[2];
     uint32_t bits[4];
     double totals[5];
+    FarList far;
 };
 
 constexpr size_t kWsHeaderBytes = 1024;

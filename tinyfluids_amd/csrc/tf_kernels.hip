@@ -82,6 +82,9 @@ __device__ __forceinline__ T *sgpr_ptr(T *p) {
 template <int TX>
 struct TileCtx {
     __amdgpu_buffer_rsrc_t rin[5], rout[5];
+    const float *in_ptr;  // 64-bit field addressing of the far-row variant
+    float *out_ptr;
+    size_t nc;
     uint32_t xoff[TX + 2];  // byte offsets of the loaded x-rows (uniform)
     uint32_t zb;            // this lane's load byte offset along z
     uint32_t zbo;           // this lane's store byte offset (z - z0)
@@ -97,17 +100,37 @@ __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t vof
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, (int)voff, (int)soff, TF_STORE_AUX);
 }
 
-template <int TX>
+// A raw buffer access is in range only if offset + 4 <= num_records, and
+// num_records is at most 0xFFFFFFFF, so the last dword of a field of exactly
+// 4 GiB is out of reach of its resource. The tiles that touch that row run
+// in a separate kernel (FAR = true) with 64-bit global addresses.
+template <int TX, bool FAR>
 __device__ __forceinline__ void load_cell(const TileCtx<TX> &T, uint32_t soff, float (&w)[5]) {
+    if constexpr (FAR) {
 #pragma unroll
-    for (int f = 0; f < 5; ++f) w[f] = buf_load(T.rin[f], T.zb, soff);
+        for (int f = 0; f < 5; ++f) w[f] = T.in_ptr[f * T.nc + ((size_t)soff + T.zb) / 4];
+    } else {
+#pragma unroll
+        for (int f = 0; f < 5; ++f) w[f] = buf_load(T.rin[f], T.zb, soff);
+    }
+}
+
+template <int TX, bool FAR>
+__device__ __forceinline__ void store_cell(const TileCtx<TX> &T, uint32_t soff, const float (&w)[5]) {
+    if constexpr (FAR) {
+#pragma unroll
+        for (int f = 0; f < 5; ++f) T.out_ptr[f * T.nc + ((size_t)soff + T.zbo) / 4] = w[f];
+    } else {
+#pragma unroll
+        for (int f = 0; f < 5; ++f) buf_store(T.rout[f], T.zbo, soff, w[f]);
+    }
 }
 
 // One row of the stream: X sweep of row y0-1+r, its Y flux against the
 // previous row (qa/fa in), the Y update + Z sweep + store of the previous
 // row, and the reload of every loaded column with row r+1. Writes the new
 // Y-stream state to qb/fb.
-template <int TX>
+template <int TX, bool FAR>
 __device__ __forceinline__ void stream_row(const TileCtx<TX> &T, const Gas &g, int r, float (&cur)[TX + 2][5],
                                            const QS (&qa)[TX], const float (&fa)[TX][5], QS (&qb)[TX],
                                            float (&fb)[TX][5], uint32_t &vmax) {
@@ -115,15 +138,15 @@ __device__ __forceinline__ void stream_row(const TileCtx<TX> &T, const Gas &g, i
     const int ygn = T.open_bc ? open_clamp(wrap1(yn, T.ny), T.ny) : wrap1(yn, T.ny);
     const uint32_t yoffn = (uint32_t)__builtin_amdgcn_readfirstlane(ygn) * T.row_b;
     Q qx0 = q_from_prim_x(cur[0], g);
-    load_cell(T, T.xoff[0] + yoffn, cur[0]);
+    load_cell<TX, FAR>(T, T.xoff[0] + yoffn, cur[0]);
     Q qx1 = q_from_prim_x(cur[1], g);
-    load_cell(T, T.xoff[1] + yoffn, cur[1]);
+    load_cell<TX, FAR>(T, T.xoff[1] + yoffn, cur[1]);
     float fx0[5];
     hll(qx0, qx1, fx0);
 #pragma unroll
     for (int s = 0; s < TX; ++s) {
         const Q qx2 = q_from_prim_x(cur[s + 2], g);
-        load_cell(T, T.xoff[s + 2] + yoffn, cur[s + 2]);
+        load_cell<TX, FAR>(T, T.xoff[s + 2] + yoffn, cur[s + 2]);
         float fx1[5];
         hll(qx1, qx2, fx1);
         float U1[5];
@@ -154,11 +177,8 @@ __device__ __forceinline__ void stream_row(const TileCtx<TX> &T, const Gas &g, i
             if (T.zout && s < T.nxo) {
                 const uint32_t so = (uint32_t)(T.x0 + s) * T.plane_b + (uint32_t)(T.y0 + r - 2) * T.row_b +
                                     4u * (uint32_t)T.z0;
-                buf_store(T.rout[0], T.zbo, so, w3.r);
-                buf_store(T.rout[1], T.zbo, so, w3.v[0]);
-                buf_store(T.rout[2], T.zbo, so, w3.v[1]);
-                buf_store(T.rout[3], T.zbo, so, w3.v[2]);
-                buf_store(T.rout[4], T.zbo, so, w3.p);
+                const float wo[5] = {w3.r, w3.v[0], w3.v[1], w3.v[2], w3.p};
+                store_cell<TX, FAR>(T, so, wo);
                 const float sp = signal_speed(w3.v, sound(w3.p, w3.rinv, g));
                 vmax = max(vmax, __float_as_uint(sp));
             }
@@ -175,7 +195,7 @@ __device__ __forceinline__ float sgpr_f(float v) {
     return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
 }
 
-template <int TX>
+template <int TX, bool FAR>
 __device__ __forceinline__ uint32_t fused_tile(const float *in_, float *out_, int nx, int ny, int nz, int open_bc,
                                                const Gas &g_in, int xt, int zt, int yc, int ly) {
     const Gas g{sgpr_f(g_in.gamma), sgpr_f(g_in.gm1), sgpr_f(g_in.inv_gm1), sgpr_f(g_in.k)};
@@ -194,6 +214,9 @@ __device__ __forceinline__ uint32_t fused_tile(const float *in_, float *out_, in
     {
         const float *pin = sgpr_ptr(in_);
         float *pout = sgpr_ptr(out_);
+        T.in_ptr = pin;
+        T.out_ptr = pout;
+        T.nc = nc;
 #pragma unroll
         for (int f = 0; f < 5; ++f) {
             T.rin[f] = __builtin_amdgcn_make_buffer_rsrc((void *)(pin + f * nc), (short)0, -1, 0x00020000);
@@ -225,7 +248,7 @@ __device__ __forceinline__ uint32_t fused_tile(const float *in_, float *out_, in
     {
         const uint32_t yoff = (uint32_t)__builtin_amdgcn_readfirstlane(cell_index(T.y0 - 1, ny, open_bc)) * T.row_b;
 #pragma unroll
-        for (int j = 0; j < TX + 2; ++j) load_cell(T, T.xoff[j] + yoff, cur[j]);
+        for (int j = 0; j < TX + 2; ++j) load_cell<TX, FAR>(T, T.xoff[j] + yoff, cur[j]);
     }
     // Y-stream state of the previous row, in two alternating sets so that
     // rows hand it over without register copies.
@@ -234,10 +257,10 @@ __device__ __forceinline__ uint32_t fused_tile(const float *in_, float *out_, in
     uint32_t vmax = 0;
     int r = 0;
     for (; r + 1 < T.nrows; r += 2) {
-        stream_row<TX>(T, g, r, cur, qA, fA, qB, fB, vmax);
-        stream_row<TX>(T, g, r + 1, cur, qB, fB, qA, fA, vmax);
+        stream_row<TX, FAR>(T, g, r, cur, qA, fA, qB, fB, vmax);
+        stream_row<TX, FAR>(T, g, r + 1, cur, qB, fB, qA, fA, vmax);
     }
-    if (r < T.nrows) stream_row<TX>(T, g, r, cur, qA, fA, qB, fB, vmax);
+    if (r < T.nrows) stream_row<TX, FAR>(T, g, r, cur, qA, fA, qB, fB, vmax);
     return vmax;
 }
 
@@ -281,22 +304,49 @@ static inline unsigned xcd_grid(int total_waves) {
     return (unsigned)(((nblocks + 7) / 8) * 8);
 }
 
-// Fused step of one grid with an explicit dt (the _evolve_state entry point).
 // Waves per SIMD the register allocation must allow: the stream state grows
 // with TX, and below two resident waves nothing hides the load latency.
 template <int TX>
 constexpr int waves_per_simd() { return TX <= 2 ? 4 : (TX <= 4 ? 3 : (TX <= 6 ? 2 : 1)); }
 
+// Does tile (xt, yc) load or store the last (x, y) row? Loaded x-rows are
+// x0-1 .. x0+TX (the first tile's left halo wraps to nx-1), loaded y-rows
+// y0-1 .. min(y0+ly, ny) (the first chunk's lower halo wraps to ny-1).
 template <int TX>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_per_simd<TX>()))) void k_step_direct(SimDev S, const float *__restrict__ in,
-                                                        float *__restrict__ out, const double *__restrict__ dt,
-                                                        double dx, uint32_t *__restrict__ bits_out) {
+__device__ __forceinline__ bool touches_last_row(const SimDev &S, int xt, int yc) {
+    const bool xr = xt == 0 || xt * TX + TX >= S.nx - 1;
+    const bool yr = yc == 0 || yc * S.ly + S.ly >= S.ny - 1;
+    return xr && yr;
+}
+
+// Fused step of one grid with an explicit dt (the _evolve_state entry point).
+template <int TX>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_per_simd<TX>()))) void k_step_direct(
+    SimDev S, const float *__restrict__ in, float *__restrict__ out, const double *__restrict__ dt, double dx,
+    uint32_t *__restrict__ bits_out) {
     const int w = xcd_wave_index(S.waves);
     if (w < 0) return;
     Gas g{S.gamma, S.gm1, S.inv_gm1, (float)(*dt / dx)};
     int xt, zt, yc;
     decode_tile(w, S.ntz, S.ntx, xt, zt, yc);
-    uint32_t v = fused_tile<TX>(in, out, S.nx, S.ny, S.nz, S.open_bc, g, xt, zt, yc, S.ly);
+    if (S.far && touches_last_row<TX>(S, xt, yc)) return;  // k_step_direct_far does it
+    uint32_t v = fused_tile<TX, false>(in, out, S.nx, S.ny, S.nz, S.open_bc, g, xt, zt, yc, S.ly);
+    if (bits_out) {
+        v = wave_umax(v);
+        if (lane_id() == 0) atomicMax(bits_out, v);
+    }
+}
+
+// The far tiles of k_step_direct: one wave per (listed tile, z-tile).
+template <int TX>
+__global__ __launch_bounds__(kBlock) void k_step_direct_far(SimDev S, FarList F, const float *__restrict__ in,
+                                                            float *__restrict__ out, const double *__restrict__ dt,
+                                                            double dx, uint32_t *__restrict__ bits_out) {
+    const int w = blockIdx.x * (kBlock / kWave) + wave_id_uniform();
+    if (w >= F.n * S.ntz) return;
+    const FarTile &t = F.t[w / S.ntz];
+    Gas g{S.gamma, S.gm1, S.inv_gm1, (float)(*dt / dx)};
+    uint32_t v = fused_tile<TX, true>(in, out, S.nx, S.ny, S.nz, S.open_bc, g, t.xt, w % S.ntz, t.yc, S.ly);
     if (bits_out) {
         v = wave_umax(v);
         if (lane_id() == 0) atomicMax(bits_out, v);
@@ -306,8 +356,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_pe
 // Fused step of every active simulation of a table; simulation s owns waves
 // [prefix[s], prefix[s+1]). Input/output buffers follow the step parity.
 template <int TX>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_per_simd<TX>()))) void k_step_sims(const SimDev *__restrict__ sims, const int *__restrict__ prefix,
-                                                      int nsims) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_per_simd<TX>()))) void k_step_sims(
+    const SimDev *__restrict__ sims, const int *__restrict__ prefix, int nsims) {
     const int w = xcd_wave_index(prefix[nsims]);
     if (w < 0) return;
     int lo = 0, hi = nsims;
@@ -322,7 +372,29 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_pe
     Gas g{S.gamma, S.gm1, S.inv_gm1, C->k};
     int xt, zt, yc;
     decode_tile(w - prefix[lo], S.ntz, S.ntx, xt, zt, yc);
-    uint32_t v = fused_tile<TX>(S.buf[par], S.buf[par ^ 1], S.nx, S.ny, S.nz, S.open_bc, g, xt, zt, yc, S.ly);
+    if (S.far && touches_last_row<TX>(S, xt, yc)) return;  // k_step_sims_far does it
+    uint32_t v = fused_tile<TX, false>(S.buf[par], S.buf[par ^ 1], S.nx, S.ny, S.nz, S.open_bc, g, xt, zt, yc, S.ly);
+    v = wave_umax(v);
+    if (lane_id() == 0) atomicMax(&C->max_bits[par ^ 1], v);
+}
+
+// The far tiles of k_step_sims: `far` lists (simulation, x-tile, y-chunk);
+// one wave per (listed tile, z-tile), max_ntz waves per entry.
+template <int TX>
+__global__ __launch_bounds__(kBlock) void k_step_sims_far(const SimDev *__restrict__ sims,
+                                                          const FarTile *__restrict__ far, int nfar, int max_ntz) {
+    const int w = blockIdx.x * (kBlock / kWave) + wave_id_uniform();
+    if (w >= nfar * max_ntz) return;
+    const FarTile t = far[w / max_ntz];
+    const SimDev &S = sims[t.sim];
+    const int zt = w % max_ntz;
+    if (zt >= S.ntz) return;
+    Ctrl *C = S.ctrl;
+    if (!C->active) return;
+    const int par = (int)(C->n_iter & 1);
+    Gas g{S.gamma, S.gm1, S.inv_gm1, C->k};
+    uint32_t v = fused_tile<TX, true>(S.buf[par], S.buf[par ^ 1], S.nx, S.ny, S.nz, S.open_bc, g, t.xt, zt, t.yc,
+                                      S.ly);
     v = wave_umax(v);
     if (lane_id() == 0) atomicMax(&C->max_bits[par ^ 1], v);
 }
@@ -547,6 +619,34 @@ hipError_t launch_step_direct(int tx, const SimDev &S, const float *in, float *o
         case 4: hipLaunchKernelGGL(k_step_direct<4>, grid, block, 0, st, S, in, out, dt, dx, bits); break;
         case 6: hipLaunchKernelGGL(k_step_direct<6>, grid, block, 0, st, S, in, out, dt, dx, bits); break;
         case 8: hipLaunchKernelGGL(k_step_direct<8>, grid, block, 0, st, S, in, out, dt, dx, bits); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_step_direct_far(int tx, const SimDev &S, const FarList &F, const float *in, float *out,
+                                  const double *dt, double dx, uint32_t *bits, hipStream_t st) {
+    if (F.n <= 0) return hipSuccess;
+    const dim3 grid((unsigned)((F.n * S.ntz + 3) / 4)), block(kBlock);
+    switch (tx) {
+        case 2: hipLaunchKernelGGL(k_step_direct_far<2>, grid, block, 0, st, S, F, in, out, dt, dx, bits); break;
+        case 4: hipLaunchKernelGGL(k_step_direct_far<4>, grid, block, 0, st, S, F, in, out, dt, dx, bits); break;
+        case 6: hipLaunchKernelGGL(k_step_direct_far<6>, grid, block, 0, st, S, F, in, out, dt, dx, bits); break;
+        case 8: hipLaunchKernelGGL(k_step_direct_far<8>, grid, block, 0, st, S, F, in, out, dt, dx, bits); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_step_sims_far(int tx, const SimDev *sims, const FarTile *far, int nfar, int max_ntz,
+                                hipStream_t st) {
+    if (nfar <= 0) return hipSuccess;
+    const dim3 grid((unsigned)((nfar * max_ntz + 3) / 4)), block(kBlock);
+    switch (tx) {
+        case 2: hipLaunchKernelGGL(k_step_sims_far<2>, grid, block, 0, st, sims, far, nfar, max_ntz); break;
+        case 4: hipLaunchKernelGGL(k_step_sims_far<4>, grid, block, 0, st, sims, far, nfar, max_ntz); break;
+        case 6: hipLaunchKernelGGL(k_step_sims_far<6>, grid, block, 0, st, sims, far, nfar, max_ntz); break;
+        case 8: hipLaunchKernelGGL(k_step_sims_far<8>, grid, block, 0, st, sims, far, nfar, max_ntz); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
