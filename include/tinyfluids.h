@@ -174,6 +174,12 @@ int tf_batch_step_graph(tf_batch *b, int steps, void *stream);
  * duration of the stencil launches alone (without the dt updates). */
 int tf_batch_step_timed(tf_batch *b, int steps, float *stencil_ms, void *stream);
 
+/* Process-wide default for plans built afterwards: 1 (default) lets a plan
+ * run small grids (nz <= 256) with the line tiling, whole periodic z-lines
+ * per workgroup, where that keeps more lanes busy than 62-column tiles;
+ * 0 uses the 62-column tiling for every simulation. Results are identical. */
+int tf_set_line_mode(int enabled);
+
 /* Copies every simulation's status to host memory (synchronous). */
 int tf_batch_status(tf_batch *b, tf_sim_status *host_out, void *stream);
 

@@ -109,3 +109,42 @@ def test_server_respects_hbm_budget():
     assert all(j.steps_done == 6 for j in rep.jobs)
     s = rep.summary()
     assert s["p99_latency_s"] >= s["p50_latency_s"] > 0
+
+
+LINE_SHAPES = [((5, 6, 8), False), ((7, 5, 16), True), ((6, 4, 24), False), ((4, 9, 32), False),
+               ((3, 5, 48), True), ((6, 3, 64), False), ((3, 4, 100), False), ((4, 3, 128), True),
+               ((3, 3, 192), False), ((2, 3, 256), False), ((9, 2, 20), False)]
+
+
+@pytest.mark.parametrize("line_mode", [1, 0])
+@pytest.mark.parametrize("tile_x", [2, 4, 8])
+def test_line_tiling_bitwise(line_mode, tile_x):
+    """Batched plans with whole-z-line workgroups (W = 1..4 waves, several
+    x-groups per wave for nz <= 32) give the plain loop's results bit for bit."""
+    import tinyfluids_amd._lib as L
+    from tinyfluids_amd.batch import Batch, Simulation
+    from tinyfluids_amd.time_integration import run_steps
+    rng = np.random.default_rng(21)
+    sims, init = [], []
+    for (shape, hb) in LINE_SHAPES:
+        s = np.empty((5,) + shape)
+        s[0] = 1.0 + 0.5 * rng.uniform(0, 1, shape)
+        s[1:4] = 0.5 * rng.uniform(-1, 1, (3,) + shape)
+        s[4] = 1.0 + 0.5 * rng.uniform(0, 1, shape)
+        x = torch.from_numpy(s.astype(np.float32)).to(DEV)
+        init.append((x, hb))
+        sims.append(Simulation(x, 0.05, 1.4, max_steps=4, handle_boundaries=hb, device=DEV))
+    lib = L.load()
+    try:
+        lib.tf_set_line_mode(line_mode)
+        b = Batch(sims, tile_x=tile_x)
+    finally:
+        lib.tf_set_line_mode(1)
+    for s in sims:
+        s.reset()
+    b.step_graph(4)
+    st = b.status()
+    for i, ((x, hb), sim) in enumerate(zip(init, sims)):
+        ref, _, _ = run_steps(x, 0.05, 1.4, 4, handle_boundaries=hb)
+        assert st[i].n_iter == 4
+        assert torch.equal(sim.state(st[i]), ref), (LINE_SHAPES[i], line_mode, tile_x)

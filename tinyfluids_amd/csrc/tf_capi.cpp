@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdarg>
 #include <cmath>
 #include <cstdio>
@@ -25,6 +26,8 @@ hipError_t launch_step_direct_far(int tx, const SimDev &S, const FarList &F, con
                                   const double *dt, double dx, uint32_t *bits, hipStream_t st);
 hipError_t launch_step_sims_far(int tx, const SimDev *sims, const FarTile *far, int nfar, int max_ntz,
                                 hipStream_t st);
+hipError_t launch_line_sims(int tx, int w, const SimDev *sims, const int *prefix, int nsims, int nblocks,
+                            hipStream_t st);
 hipError_t launch_finalize_sims(const SimDev *sims, int nsims, hipStream_t st);
 hipError_t launch_finalize_one(Ctrl *C, int first, hipStream_t st);
 hipError_t launch_ctrl_reset(Ctrl *C, double dx, double c_cfl, double t_final, int64_t max_steps, hipStream_t st);
@@ -398,6 +401,14 @@ int tf_autotune_evolve(const float *in, float *out, int nx, int ny, int nz, doub
 }
 
 // ------------------------------------------------------------------ batched engine
+// A class of simulations launched together: halo tiling (w == 0, prefix over
+// waves) or line tiling with w waves per workgroup (prefix over workgroups).
+struct SimClass {
+    int w = 0;
+    int begin = 0, count = 0, total = 0;
+    int *d_prefix = nullptr;
+};
+
 struct tf_batch {
     int nsims = 0;
     int tx = 4;
@@ -405,6 +416,7 @@ struct tf_batch {
     SimDev *d_sims = nullptr;
     int *d_prefix = nullptr;
     Ctrl *d_status = nullptr;
+    SimClass cls[5];
     std::vector<SimDev> h_sims;  // device table order
     std::vector<int> h_prefix;
     std::vector<int> order;      // h_sims[k] is the caller's simulation order[k]
@@ -421,8 +433,30 @@ static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 size_t tf_batch_workspace_bytes(int nsims) {
     if (nsims < 0) nsims = 0;
-    return align256((size_t)nsims * sizeof(SimDev)) + align256((size_t)(nsims + 1) * sizeof(int)) +
+    return align256((size_t)nsims * sizeof(SimDev)) + align256((size_t)(nsims + 5) * sizeof(int)) +
            align256((size_t)nsims * sizeof(Ctrl)) + align256((size_t)nsims * kMaxFarPerSim * sizeof(FarTile));
+}
+
+namespace {
+std::atomic<int> g_line_mode{1};
+
+// Line tiling of a z-line of nz cells: w waves per workgroup, and with
+// w == 1 as many x-column groups as whole lines fit in a wave.
+void line_shape(int nz, int &w, int &groups) {
+    w = (nz + 63) / 64;
+    groups = w == 1 ? std::max(1, 64 / nz) : 1;
+}
+double halo_lane_use(int nz) { return (double)nz / (64.0 * ((nz + kZT - 1) / kZT)); }
+double line_lane_use(int nz) {
+    int w, g;
+    line_shape(nz, w, g);
+    return (double)(g * nz) / (64.0 * w);
+}
+}  // namespace
+
+int tf_set_line_mode(int enabled) {
+    g_line_mode.store(enabled ? 1 : 0);
+    return TF_OK;
 }
 
 int tf_sim_reset(const tf_sim_desc *sim, void *stream) {
@@ -456,6 +490,7 @@ int tf_batch_create(tf_batch **out, const tf_sim_desc *sims, int nsims, void *wo
         if (int e = check_open_bc(d.handle_boundaries, d.nx, d.ny, d.nz)) return e;
         TF_REQUIRE(d.buf0 && d.buf1 && d.ctrl, "null device pointer in sim %d", i);
     }
+    const bool line_on = g_line_mode.load() != 0;
     auto build = [&](int ly_cap, std::vector<SimDev> &tab) {
         long total = 0;
         for (int i = 0; i < nsims; ++i) {
@@ -465,6 +500,15 @@ int tf_batch_create(tf_batch **out, const tf_sim_desc *sims, int nsims, void *wo
             S.buf[0] = d.buf0;
             S.buf[1] = d.buf1;
             S.ctrl = reinterpret_cast<Ctrl *>(d.ctrl);
+            if (line_on && !S.far && d.nz <= 256 && line_lane_use(d.nz) > halo_lane_use(d.nz) + 1e-9) {
+                int w, groups;
+                line_shape(d.nz, w, groups);
+                S.line_w = w;
+                S.groups = groups;
+                S.ntx = (d.nx + groups * tx - 1) / (groups * tx);
+                S.ntz = 1;
+                S.waves = S.ntx * S.nyc * w;
+            }
             tab[i] = S;
             total += S.waves;
         }
@@ -494,22 +538,38 @@ int tf_batch_create(tf_batch **out, const tf_sim_desc *sims, int nsims, void *wo
     b->order.resize(nsims);
     for (int i = 0; i < nsims; ++i) b->order[i] = i;
     std::stable_sort(b->order.begin(), b->order.end(), [&](int a, int c) {
+        if (tab[a].line_w != tab[c].line_w) return tab[a].line_w < tab[c].line_w;
         const long la = (long)std::min(tab[a].ly, tab[a].ny), lc = (long)std::min(tab[c].ly, tab[c].ny);
         return la > lc;
     });
     b->h_sims.resize(nsims);
-    b->h_prefix.resize(nsims + 1);
-    b->h_prefix[0] = 0;
-    for (int k = 0; k < nsims; ++k) {
-        b->h_sims[k] = tab[b->order[k]];
-        b->h_prefix[k + 1] = b->h_prefix[k] + b->h_sims[k].waves;
+    for (int k = 0; k < nsims; ++k) b->h_sims[k] = tab[b->order[k]];
+    // classes: contiguous table ranges, each with its own prefix (waves or workgroups)
+    b->h_prefix.clear();
+    b->total_waves = 0;
+    std::vector<int> cls_prefix_at(5, 0);
+    for (int c = 0, k = 0; c < 5; ++c) {
+        SimClass &C = b->cls[c];
+        C.w = c;
+        C.begin = k;
+        cls_prefix_at[c] = (int)b->h_prefix.size();
+        b->h_prefix.push_back(0);
+        while (k < nsims && b->h_sims[k].line_w == c) {
+            const SimDev &S = b->h_sims[k];
+            const int units = c == 0 ? S.waves : S.ntx * S.nyc;
+            b->h_prefix.push_back(b->h_prefix.back() + units);
+            b->total_waves += S.waves;
+            ++k;
+        }
+        C.count = k - C.begin;
+        C.total = b->h_prefix.back();
     }
-    b->total_waves = b->h_prefix[nsims];
     char *p = static_cast<char *>(workspace);
     b->d_sims = reinterpret_cast<SimDev *>(p);
     p += align256((size_t)nsims * sizeof(SimDev));
     b->d_prefix = reinterpret_cast<int *>(p);
-    p += align256((size_t)(nsims + 1) * sizeof(int));
+    for (int c = 0; c < 5; ++c) b->cls[c].d_prefix = b->d_prefix + cls_prefix_at[c];
+    p += align256((size_t)(nsims + 5) * sizeof(int));
     b->d_status = reinterpret_cast<Ctrl *>(p);
     p += align256((size_t)nsims * sizeof(Ctrl));
     b->d_far = reinterpret_cast<FarTile *>(p);
@@ -523,7 +583,8 @@ int tf_batch_create(tf_batch **out, const tf_sim_desc *sims, int nsims, void *wo
     hipStream_t st = S_(stream);
     hipError_t e = hipMemcpyAsync(b->d_sims, b->h_sims.data(), nsims * sizeof(SimDev), hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(b->d_prefix, b->h_prefix.data(), (nsims + 1) * sizeof(int), hipMemcpyHostToDevice, st);
+        e = hipMemcpyAsync(b->d_prefix, b->h_prefix.data(), b->h_prefix.size() * sizeof(int), hipMemcpyHostToDevice,
+                           st);
     if (e == hipSuccess && b->nfar > 0)
         e = hipMemcpyAsync(b->d_far, far.data(), far.size() * sizeof(FarTile), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -544,10 +605,23 @@ int tf_batch_destroy(tf_batch *b) {
     return TF_OK;
 }
 
+// One batched stencil step: the halo-tiled class, each line-tiled class, then
+// the far-row tiles (all before the dt update).
+static hipError_t launch_stencils(tf_batch *b, hipStream_t st) {
+    const SimClass &H = b->cls[0];
+    hipError_t e = hipSuccess;
+    if (H.count > 0) e = launch_step_sims(b->tx, b->d_sims + H.begin, H.d_prefix, H.count, H.total, st);
+    for (int c = 1; c < 5 && e == hipSuccess; ++c) {
+        const SimClass &L = b->cls[c];
+        if (L.count > 0) e = launch_line_sims(b->tx, c, b->d_sims + L.begin, L.d_prefix, L.count, L.total, st);
+    }
+    if (e == hipSuccess) e = launch_step_sims_far(b->tx, b->d_sims, b->d_far, b->nfar, b->far_max_ntz, st);
+    return e;
+}
+
 static int enqueue_steps(tf_batch *b, int steps, hipStream_t st) {
     for (int i = 0; i < steps; ++i) {
-        TF_HIP(launch_step_sims(b->tx, b->d_sims, b->d_prefix, b->nsims, b->total_waves, st));
-        TF_HIP(launch_step_sims_far(b->tx, b->d_sims, b->d_far, b->nfar, b->far_max_ntz, st));
+        TF_HIP(launch_stencils(b, st));
         TF_HIP(launch_finalize_sims(b->d_sims, b->nsims, st));
     }
     return TF_OK;
@@ -603,8 +677,7 @@ int tf_batch_step_timed(tf_batch *b, int steps, float *stencil_ms, void *stream)
     }
     for (int i = 0; i < steps && rc == TF_OK; ++i) {
         hipError_t e = hipEventRecord(ev[2 * i], st);
-        if (e == hipSuccess) e = launch_step_sims(b->tx, b->d_sims, b->d_prefix, b->nsims, b->total_waves, st);
-        if (e == hipSuccess) e = launch_step_sims_far(b->tx, b->d_sims, b->d_far, b->nfar, b->far_max_ntz, st);
+        if (e == hipSuccess) e = launch_stencils(b, st);
         if (e == hipSuccess) e = hipEventRecord(ev[2 * i + 1], st);
         if (e == hipSuccess) e = launch_finalize_sims(b->d_sims, b->nsims, st);
         if (e != hipSuccess) rc = fail(TF_ERR_HIP, "timed step: %s", hipGetErrorString(e));

@@ -41,7 +41,8 @@ struct SimDev {
     int ntx, ntz, nyc, ly;  // tiles along x (of TX), z (of kZT), y-chunks and their length
     int waves;              // ntx * ntz * nyc
     int far;                // fields of exactly 4 GiB: tiles touching the last (x, y) row run in k_step_far
-    int pad2[2];
+    int line_w;             // 0: halo tiling (waves of 62 z-columns); W >= 1: line tiling, W waves per z-line
+    int groups;             // line tiling: x-column groups per wave (nz <= 32 packs several)
 };
 
 // A tile handled by the far-row kernel: simulation index, x-tile, y-chunk.

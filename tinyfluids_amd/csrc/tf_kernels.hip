@@ -50,13 +50,6 @@ __device__ __forceinline__ uint32_t wave_umax(uint32_t v) {
 typedef __attribute__((address_space(1))) const float gcfloat;
 typedef __attribute__((address_space(1))) float gfloat;
 
-// Sweep quantities along x of a loaded primitive cell.
-__device__ __forceinline__ Q q_from_prim_x(const float (&w)[5], const Gas &g) {
-    float U[5], rinv;
-    prim_to_cons(w, g, U, rinv);
-    const float v[3] = {w[1], w[2], w[3]};
-    return make_q<1>(U, v, w[4], rinv, g);
-}
 
 // Loads the 5 fields of one cell: `col` is the wave-uniform address of the
 // cell row's first field, `zb` the lane's byte offset along z (< 4 GiB), so
@@ -74,79 +67,171 @@ __device__ __forceinline__ T *sgpr_ptr(T *p) {
     return (T *)(((uint64_t)hi << 32) | lo);
 }
 
-// State access of one tile. Each of the 5 fields of the input and output
-// has a buffer resource in SGPRs; a cell row is a 32-bit scalar byte offset
-// (x/y position) plus the lane's byte offset along z, so loads and stores
-// need no per-lane address arithmetic. Fields must be < 4 GiB
-// (nx*ny*nz <= 2^30); larger grids are rejected on the host.
-template <int TX>
-struct TileCtx {
-    __amdgpu_buffer_rsrc_t rin[5], rout[5];
-    const float *in_ptr;  // 64-bit field addressing of the far-row variant
-    float *out_ptr;
-    size_t nc;
-    uint32_t xoff[TX + 2];  // byte offsets of the loaded x-rows (uniform)
-    uint32_t zb;            // this lane's load byte offset along z
-    uint32_t zbo;           // this lane's store byte offset (z - z0)
-    int nz, ny, y0, x0, z0, nxo, nrows, open_bc;
-    uint32_t plane_b, row_b;  // bytes per x-plane and per y-row of one field
-    bool zout;
-};
-
 __device__ __forceinline__ float buf_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
 }
 __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, float v) {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, (int)voff, (int)soff, TF_STORE_AUX);
 }
-
-// A raw buffer access is in range only if offset + 4 <= num_records, and
-// num_records is at most 0xFFFFFFFF, so the last dword of a field of exactly
-// 4 GiB is out of reach of its resource. The tiles that touch that row run
-// in a separate kernel (FAR = true) with 64-bit global addresses.
-template <int TX, bool FAR>
-__device__ __forceinline__ void load_cell(const TileCtx<TX> &T, uint32_t soff, float (&w)[5]) {
-    if constexpr (FAR) {
-#pragma unroll
-        for (int f = 0; f < 5; ++f) w[f] = T.in_ptr[f * T.nc + ((size_t)soff + T.zb) / 4];
-    } else {
-#pragma unroll
-        for (int f = 0; f < 5; ++f) w[f] = buf_load(T.rin[f], T.zb, soff);
-    }
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t field_rsrc(const float *field) {
+    return __builtin_amdgcn_make_buffer_rsrc((void *)field, (short)0, -1, 0x00020000);
+}
+__device__ __forceinline__ float sgpr_f(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
 }
 
+// ---------------------------------------------------------------------------
+// Tile access policies. A policy gives the stream (stream_row) its loads,
+// its z-neighbour exchange and its stores; the arithmetic is shared.
+//
+// HaloCtx: one wave per tile of TX x-columns x 62 z-columns (lanes 0 and 63
+// are z-halo columns recomputed by the wave), neighbours by DPP lane shifts.
+// Each of the 5 fields of input and output has a buffer resource in SGPRs; a
+// cell row is a 32-bit scalar byte offset (x/y position) plus the lane's
+// byte offset along z, so loads and stores need no per-lane address
+// arithmetic. A raw buffer access is in range only if offset + 4 <=
+// num_records <= 0xFFFFFFFF, so the last dword of a field of exactly 4 GiB
+// is out of reach of its resource: the tiles that touch that row run in a
+// separate kernel with FAR = true (64-bit global addresses).
+// ---------------------------------------------------------------------------
 template <int TX, bool FAR>
-__device__ __forceinline__ void store_cell(const TileCtx<TX> &T, uint32_t soff, const float (&w)[5]) {
-    if constexpr (FAR) {
+struct HaloCtx {
+    __amdgpu_buffer_rsrc_t rin[5], rout[5];
+    const float *in_ptr;
+    float *out_ptr;
+    size_t nc;
+    uint32_t xoff[TX + 2];  // byte offsets of the loaded x-rows (uniform)
+    uint32_t zb;            // this lane's load byte offset along z
+    uint32_t zbo;           // this lane's store byte offset (z - z0)
+    int ny, y0, x0, z0, nxo, nrows, open_bc;
+    uint32_t plane_b, row_b;  // bytes per x-plane and per y-row of one field
+    bool zout;
+
+    __device__ __forceinline__ void load(int j, uint32_t yoff, float (&w)[5]) const {
+        const uint32_t soff = xoff[j] + yoff;
+        if constexpr (FAR) {
 #pragma unroll
-        for (int f = 0; f < 5; ++f) T.out_ptr[f * T.nc + ((size_t)soff + T.zbo) / 4] = w[f];
-    } else {
+            for (int f = 0; f < 5; ++f) w[f] = in_ptr[f * nc + ((size_t)soff + zb) / 4];
+        } else {
 #pragma unroll
-        for (int f = 0; f < 5; ++f) buf_store(T.rout[f], T.zbo, soff, w[f]);
+            for (int f = 0; f < 5; ++f) w[f] = buf_load(rin[f], zb, soff);
+        }
     }
+    __device__ __forceinline__ void left(const Q &q, Q &ql, int) const {
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            ql.U[i] = from_left(q.U[i]);
+            ql.F[i] = from_left(q.F[i]);
+        }
+        ql.up = from_left(q.up);
+        ql.um = from_left(q.um);
+    }
+    __device__ __forceinline__ void right(const float (&fz)[5], float (&fzr)[5], int) const {
+#pragma unroll
+        for (int i = 0; i < 5; ++i) fzr[i] = from_right(fz[i]);
+    }
+    __device__ __forceinline__ bool out_ok(int s) const { return zout && s < nxo; }
+    __device__ __forceinline__ void store(int s, int y, const float (&w)[5]) const {
+        const uint32_t so = (uint32_t)(x0 + s) * plane_b + (uint32_t)y * row_b + 4u * (uint32_t)z0;
+        if constexpr (FAR) {
+#pragma unroll
+            for (int f = 0; f < 5; ++f) out_ptr[f * nc + ((size_t)so + zbo) / 4] = w[f];
+        } else {
+#pragma unroll
+            for (int f = 0; f < 5; ++f) buf_store(rout[f], zbo, so, w[f]);
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// LineCtx: a workgroup of W waves holds whole periodic z-lines (nz <= 64 W),
+// so there are no z-halo columns. With W = 1 and nz <= 32 the wave packs
+// G = 64/nz groups of lanes, each on its own TX x-columns. Lane gl = (group
+// g, z); z-neighbours are exchanged through LDS with periodic indices, two
+// barriers per cell row (all waves of the group run the same rows). Loads:
+// per-lane voffset (x-row and z, fixed per tile) + scalar y-row offset.
+// ---------------------------------------------------------------------------
+template <int TX, int W>
+struct LineCtx {
+    __amdgpu_buffer_rsrc_t rin[5], rout[5];
+    uint32_t xz[TX + 2];  // per-lane byte offsets of the loaded columns (x-row, z; wrapped/clamped)
+    uint32_t xs_b;        // per-lane byte offset of the lane's own cell in slot 0 (unclamped x, z)
+    int ny, y0, nrows, open_bc;
+    uint32_t row_b, plane_b;
+    int lidx, ridx, gl;   // LDS slots: left / right neighbour, own
+    int xs0;              // first x-column of this lane's group
+    int nx;
+    bool valid;
+    float *lq;            // LDS: 12 x 64W floats per slot parity
+    float *lf;            // LDS: 5 x 64W floats per slot parity
+
+    __device__ __forceinline__ void load(int j, uint32_t yoff, float (&w)[5]) const {
+#pragma unroll
+        for (int f = 0; f < 5; ++f) w[f] = buf_load(rin[f], xz[j], yoff);
+    }
+    __device__ __forceinline__ void left(const Q &q, Q &ql, int s) const {
+        float *b = lq + (s & 1) * 12 * 64 * W;
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            b[i * 64 * W + gl] = q.U[i];
+            b[(5 + i) * 64 * W + gl] = q.F[i];
+        }
+        b[10 * 64 * W + gl] = q.up;
+        b[11 * 64 * W + gl] = q.um;
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            ql.U[i] = b[i * 64 * W + lidx];
+            ql.F[i] = b[(5 + i) * 64 * W + lidx];
+        }
+        ql.up = b[10 * 64 * W + lidx];
+        ql.um = b[11 * 64 * W + lidx];
+    }
+    __device__ __forceinline__ void right(const float (&fz)[5], float (&fzr)[5], int s) const {
+        float *b = lf + (s & 1) * 5 * 64 * W;
+#pragma unroll
+        for (int i = 0; i < 5; ++i) b[i * 64 * W + gl] = fz[i];
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 5; ++i) fzr[i] = b[i * 64 * W + ridx];
+    }
+    __device__ __forceinline__ bool out_ok(int s) const { return valid && xs0 + s < nx; }
+    __device__ __forceinline__ void store(int s, int y, const float (&w)[5]) const {
+        const uint32_t so = (uint32_t)y * row_b + (uint32_t)s * plane_b;
+#pragma unroll
+        for (int f = 0; f < 5; ++f) buf_store(rout[f], xs_b, so, w[f]);
+    }
+};
+
+// Sweep quantities along x of a loaded primitive cell.
+__device__ __forceinline__ Q q_from_prim_x(const float (&w)[5], const Gas &g) {
+    float U[5], rinv;
+    prim_to_cons(w, g, U, rinv);
+    const float v[3] = {w[1], w[2], w[3]};
+    return make_q<1>(U, v, w[4], rinv, g);
 }
 
 // One row of the stream: X sweep of row y0-1+r, its Y flux against the
 // previous row (qa/fa in), the Y update + Z sweep + store of the previous
 // row, and the reload of every loaded column with row r+1. Writes the new
 // Y-stream state to qb/fb.
-template <int TX, bool FAR>
-__device__ __forceinline__ void stream_row(const TileCtx<TX> &T, const Gas &g, int r, float (&cur)[TX + 2][5],
+template <int TX, class Ctx>
+__device__ __forceinline__ void stream_row(const Ctx &T, const Gas &g, int r, float (&cur)[TX + 2][5],
                                            const QS (&qa)[TX], const float (&fa)[TX][5], QS (&qb)[TX],
                                            float (&fb)[TX][5], uint32_t &vmax) {
     const int yn = r + 1 < T.nrows ? T.y0 + r : T.y0 + r - 1;  // the last row reloads itself
     const int ygn = T.open_bc ? open_clamp(wrap1(yn, T.ny), T.ny) : wrap1(yn, T.ny);
     const uint32_t yoffn = (uint32_t)__builtin_amdgcn_readfirstlane(ygn) * T.row_b;
     Q qx0 = q_from_prim_x(cur[0], g);
-    load_cell<TX, FAR>(T, T.xoff[0] + yoffn, cur[0]);
+    T.load(0, yoffn, cur[0]);
     Q qx1 = q_from_prim_x(cur[1], g);
-    load_cell<TX, FAR>(T, T.xoff[1] + yoffn, cur[1]);
+    T.load(1, yoffn, cur[1]);
     float fx0[5];
     hll(qx0, qx1, fx0);
 #pragma unroll
     for (int s = 0; s < TX; ++s) {
         const Q qx2 = q_from_prim_x(cur[s + 2], g);
-        load_cell<TX, FAR>(T, T.xoff[s + 2] + yoffn, cur[s + 2]);
+        T.load(s + 2, yoffn, cur[s + 2]);
         float fx1[5];
         hll(qx1, qx2, fx1);
         float U1[5];
@@ -160,25 +245,16 @@ __device__ __forceinline__ void stream_row(const TileCtx<TX> &T, const Gas &g, i
             const Prim w2 = cons_to_prim(U2, g);
             const Q q2 = make_q<3>(U2, w2.v, w2.p, w2.rinv, g);
             Q ql;
-#pragma unroll
-            for (int i = 0; i < 5; ++i) {
-                ql.U[i] = from_left(q2.U[i]);
-                ql.F[i] = from_left(q2.F[i]);
-            }
-            ql.up = from_left(q2.up);
-            ql.um = from_left(q2.um);
+            T.left(q2, ql, s);
             float fz[5], fzr[5];
             hll(ql, q2, fz);
-#pragma unroll
-            for (int i = 0; i < 5; ++i) fzr[i] = from_right(fz[i]);
+            T.right(fz, fzr, s);
             float U3[5];
             godunov(q2.U, fz, fzr, g.k, U3);
             const Prim w3 = cons_to_prim(U3, g);
-            if (T.zout && s < T.nxo) {
-                const uint32_t so = (uint32_t)(T.x0 + s) * T.plane_b + (uint32_t)(T.y0 + r - 2) * T.row_b +
-                                    4u * (uint32_t)T.z0;
+            if (T.out_ok(s)) {
                 const float wo[5] = {w3.r, w3.v[0], w3.v[1], w3.v[2], w3.p};
-                store_cell<TX, FAR>(T, so, wo);
+                T.store(s, T.y0 + r - 2, wo);
                 const float sp = signal_speed(w3.v, sound(w3.p, w3.rinv, g));
                 vmax = max(vmax, __float_as_uint(sp));
             }
@@ -189,12 +265,30 @@ __device__ __forceinline__ void stream_row(const TileCtx<TX> &T, const Gas &g, i
     }
 }
 
-// One wave: the fused X->Y->Z step on its tile. Returns the wave-local max
-// of the signal-speed bits of the cells it wrote (0 if none).
-__device__ __forceinline__ float sgpr_f(float v) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+// The stream over the rows of a tile, after the context is set up.
+template <int TX, class Ctx>
+__device__ __forceinline__ uint32_t stream_tile(const Ctx &T, const Gas &g, uint32_t yoff0) {
+    // The loaded row: each column is reloaded with the next row right after
+    // its last use, so a full row of work hides every load's latency.
+    float cur[TX + 2][5];
+#pragma unroll
+    for (int j = 0; j < TX + 2; ++j) T.load(j, yoff0, cur[j]);
+    // Y-stream state of the previous row, in two alternating sets so that
+    // rows hand it over without register copies.
+    QS qA[TX], qB[TX];
+    float fA[TX][5], fB[TX][5];
+    uint32_t vmax = 0;
+    int r = 0;
+    for (; r + 1 < T.nrows; r += 2) {
+        stream_row<TX>(T, g, r, cur, qA, fA, qB, fB, vmax);
+        stream_row<TX>(T, g, r + 1, cur, qB, fB, qA, fA, vmax);
+    }
+    if (r < T.nrows) stream_row<TX>(T, g, r, cur, qA, fA, qB, fB, vmax);
+    return vmax;
 }
 
+// One wave of the halo tiling: the fused X->Y->Z step on its tile. Returns
+// the wave-local max of the signal-speed bits of the cells it wrote.
 template <int TX, bool FAR>
 __device__ __forceinline__ uint32_t fused_tile(const float *in_, float *out_, int nx, int ny, int nz, int open_bc,
                                                const Gas &g_in, int xt, int zt, int yc, int ly) {
@@ -209,7 +303,7 @@ __device__ __forceinline__ uint32_t fused_tile(const float *in_, float *out_, in
     zt = __builtin_amdgcn_readfirstlane(zt);
     yc = __builtin_amdgcn_readfirstlane(yc);
     ly = __builtin_amdgcn_readfirstlane(ly);
-    TileCtx<TX> T;
+    HaloCtx<TX, FAR> T;
     const size_t nc = (size_t)nx * ny * nz;
     {
         const float *pin = sgpr_ptr(in_);
@@ -219,8 +313,8 @@ __device__ __forceinline__ uint32_t fused_tile(const float *in_, float *out_, in
         T.nc = nc;
 #pragma unroll
         for (int f = 0; f < 5; ++f) {
-            T.rin[f] = __builtin_amdgcn_make_buffer_rsrc((void *)(pin + f * nc), (short)0, -1, 0x00020000);
-            T.rout[f] = __builtin_amdgcn_make_buffer_rsrc((void *)(pout + f * nc), (short)0, -1, 0x00020000);
+            T.rin[f] = field_rsrc(pin + f * nc);
+            T.rout[f] = field_rsrc(pout + f * nc);
         }
     }
     const int lane = lane_id();
@@ -228,7 +322,6 @@ __device__ __forceinline__ uint32_t fused_tile(const float *in_, float *out_, in
     T.z0 = zt * kZT;
     T.y0 = yc * ly;
     T.ny = ny;
-    T.nz = nz;
     T.open_bc = open_bc;
     const int nzo = min(kZT, nz - T.z0);
     T.nrows = min(ly, ny - T.y0) + 2;
@@ -241,27 +334,64 @@ __device__ __forceinline__ uint32_t fused_tile(const float *in_, float *out_, in
 #pragma unroll
     for (int j = 0; j < TX + 2; ++j)
         T.xoff[j] = (uint32_t)__builtin_amdgcn_readfirstlane(cell_index(T.x0 - 1 + j, nx, open_bc)) * T.plane_b;
+    const uint32_t yoff0 = (uint32_t)__builtin_amdgcn_readfirstlane(cell_index(T.y0 - 1, ny, open_bc)) * T.row_b;
+    return stream_tile<TX>(T, g, yoff0);
+}
 
-    // The loaded row: each column is reloaded with the next row right after
-    // its last use, so a full row of work hides every load's latency.
-    float cur[TX + 2][5];
+// One workgroup of the line tiling (W waves, whole z-lines): returns the
+// wave-local max of the signal-speed bits of the cells this wave wrote.
+// `groups` x-column groups of nz lanes each (groups > 1 only when W == 1).
+template <int TX, int W>
+__device__ __forceinline__ uint32_t fused_line(const float *in_, float *out_, int nx, int ny, int nz, int open_bc,
+                                               const Gas &g_in, int groups, int xt, int yc, int ly, float *lq,
+                                               float *lf) {
+    const Gas g{sgpr_f(g_in.gamma), sgpr_f(g_in.gm1), sgpr_f(g_in.inv_gm1), sgpr_f(g_in.k)};
+    nx = __builtin_amdgcn_readfirstlane(nx);
+    ny = __builtin_amdgcn_readfirstlane(ny);
+    nz = __builtin_amdgcn_readfirstlane(nz);
+    open_bc = __builtin_amdgcn_readfirstlane(open_bc);
+    groups = __builtin_amdgcn_readfirstlane(groups);
+    xt = __builtin_amdgcn_readfirstlane(xt);
+    yc = __builtin_amdgcn_readfirstlane(yc);
+    ly = __builtin_amdgcn_readfirstlane(ly);
+    LineCtx<TX, W> T;
+    const size_t nc = (size_t)nx * ny * nz;
     {
-        const uint32_t yoff = (uint32_t)__builtin_amdgcn_readfirstlane(cell_index(T.y0 - 1, ny, open_bc)) * T.row_b;
+        const float *pin = sgpr_ptr(in_);
+        float *pout = sgpr_ptr(out_);
 #pragma unroll
-        for (int j = 0; j < TX + 2; ++j) load_cell<TX, FAR>(T, T.xoff[j] + yoff, cur[j]);
+        for (int f = 0; f < 5; ++f) {
+            T.rin[f] = field_rsrc(pin + f * nc);
+            T.rout[f] = field_rsrc(pout + f * nc);
+        }
     }
-    // Y-stream state of the previous row, in two alternating sets so that
-    // rows hand it over without register copies.
-    QS qA[TX], qB[TX];
-    float fA[TX][5], fB[TX][5];
-    uint32_t vmax = 0;
-    int r = 0;
-    for (; r + 1 < T.nrows; r += 2) {
-        stream_row<TX, FAR>(T, g, r, cur, qA, fA, qB, fB, vmax);
-        stream_row<TX, FAR>(T, g, r + 1, cur, qB, fB, qA, fA, vmax);
+    const int gl = (int)threadIdx.x;  // 0 .. 64 W - 1
+    int grp = gl / nz, z = gl - grp * nz;
+    T.valid = grp < groups;
+    if (!T.valid) {  // idle lanes mirror lane 0 (valid addresses, never stored)
+        grp = 0;
+        z = 0;
     }
-    if (r < T.nrows) stream_row<TX, FAR>(T, g, r, cur, qA, fA, qB, fB, vmax);
-    return vmax;
+    T.gl = gl;
+    T.lidx = grp * nz + (z == 0 ? nz - 1 : z - 1);
+    T.ridx = grp * nz + (z == nz - 1 ? 0 : z + 1);
+    T.nx = nx;
+    T.ny = ny;
+    T.y0 = yc * ly;
+    T.open_bc = open_bc;
+    T.nrows = min(ly, ny - T.y0) + 2;
+    T.row_b = 4u * (uint32_t)nz;
+    const uint32_t plane_b = T.row_b * (uint32_t)ny;
+    T.plane_b = plane_b;
+    T.xs0 = xt * groups * TX + grp * TX;
+    T.xs_b = (uint32_t)min(T.xs0, nx - 1) * plane_b + 4u * (uint32_t)z;  // stores only where xs0 + s < nx
+    const uint32_t zb = 4u * (uint32_t)cell_index(z, nz, open_bc);
+#pragma unroll
+    for (int j = 0; j < TX + 2; ++j) T.xz[j] = (uint32_t)cell_index(T.xs0 - 1 + j, nx, open_bc) * plane_b + zb;
+    T.lq = lq;
+    T.lf = lf;
+    const uint32_t yoff0 = (uint32_t)__builtin_amdgcn_readfirstlane(cell_index(T.y0 - 1, ny, open_bc)) * T.row_b;
+    return stream_tile<TX>(T, g, yoff0);
 }
 
 #ifndef TF_TILE_ORDER
@@ -395,6 +525,42 @@ __global__ __launch_bounds__(kBlock) void k_step_sims_far(const SimDev *__restri
     Gas g{S.gamma, S.gm1, S.inv_gm1, C->k};
     uint32_t v = fused_tile<TX, true>(S.buf[par], S.buf[par ^ 1], S.nx, S.ny, S.nz, S.open_bc, g, t.xt, zt, t.yc,
                                       S.ly);
+    v = wave_umax(v);
+    if (lane_id() == 0) atomicMax(&C->max_bits[par ^ 1], v);
+}
+
+// Line tiling: block b of a class of simulations with W waves per z-line;
+// simulation s owns blocks [prefix[s], prefix[s+1]) (XCD-aware order).
+// The line tiling keeps per-lane column offsets (TX + 2 VGPRs) on top of the
+// stream state, so it allows one resident wave fewer than the halo tiling.
+template <int TX>
+constexpr int line_waves_per_simd() { return TX <= 2 ? 4 : (TX <= 4 ? 2 : 1); }
+
+template <int TX, int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(line_waves_per_simd<TX>()))) void k_line_sims(
+    const SimDev *__restrict__ sims, const int *__restrict__ prefix, int nsims) {
+    __shared__ float lq[2 * 12 * 64 * W];
+    __shared__ float lf[2 * 5 * 64 * W];
+    constexpr int kXcd = 8;
+    const int nblocks = prefix[nsims];
+    const int q = (nblocks + kXcd - 1) / kXcd;
+    const int b = (int)blockIdx.x;
+    const int lb = (b % kXcd) * q + b / kXcd;
+    if (lb >= nblocks) return;  // whole workgroup: uniform, before any barrier
+    int lo = 0, hi = nsims;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (prefix[mid] <= lb) lo = mid; else hi = mid;
+    }
+    const SimDev &S = sims[lo];
+    Ctrl *C = S.ctrl;
+    if (!C->active) return;  // uniform over the workgroup
+    const int par = (int)(C->n_iter & 1);
+    Gas g{S.gamma, S.gm1, S.inv_gm1, C->k};
+    const int local = lb - prefix[lo];
+    const int xt = local % S.ntx, yc = local / S.ntx;
+    uint32_t v = fused_line<TX, W>(S.buf[par], S.buf[par ^ 1], S.nx, S.ny, S.nz, S.open_bc, g, S.groups, xt, yc,
+                                   S.ly, lq, lf);
     v = wave_umax(v);
     if (lane_id() == 0) atomicMax(&C->max_bits[par ^ 1], v);
 }
@@ -650,6 +816,32 @@ hipError_t launch_step_sims_far(int tx, const SimDev *sims, const FarTile *far, 
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+template <int W>
+static hipError_t launch_line_w(int tx, const SimDev *sims, const int *prefix, int nsims, int nblocks,
+                                hipStream_t st) {
+    const dim3 grid((unsigned)(((nblocks + 7) / 8) * 8)), block(64 * W);
+    switch (tx) {
+        case 2: hipLaunchKernelGGL((k_line_sims<2, W>), grid, block, 0, st, sims, prefix, nsims); break;
+        case 4: hipLaunchKernelGGL((k_line_sims<4, W>), grid, block, 0, st, sims, prefix, nsims); break;
+        case 6: hipLaunchKernelGGL((k_line_sims<6, W>), grid, block, 0, st, sims, prefix, nsims); break;
+        case 8: hipLaunchKernelGGL((k_line_sims<8, W>), grid, block, 0, st, sims, prefix, nsims); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_line_sims(int tx, int w, const SimDev *sims, const int *prefix, int nsims, int nblocks,
+                            hipStream_t st) {
+    if (nsims <= 0 || nblocks <= 0) return hipSuccess;
+    switch (w) {
+        case 1: return launch_line_w<1>(tx, sims, prefix, nsims, nblocks, st);
+        case 2: return launch_line_w<2>(tx, sims, prefix, nsims, nblocks, st);
+        case 3: return launch_line_w<3>(tx, sims, prefix, nsims, nblocks, st);
+        case 4: return launch_line_w<4>(tx, sims, prefix, nsims, nblocks, st);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_step_sims(int tx, const SimDev *sims, const int *prefix, int nsims, int total_waves,
