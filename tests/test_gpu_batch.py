@@ -148,3 +148,20 @@ def test_line_tiling_bitwise(line_mode, tile_x):
         ref, _, _ = run_steps(x, 0.05, 1.4, 4, handle_boundaries=hb)
         assert st[i].n_iter == 4
         assert torch.equal(sim.state(st[i]), ref), (LINE_SHAPES[i], line_mode, tile_x)
+
+
+def test_tune_batch_picks_a_measured_candidate():
+    """The batch autotuner times every candidate tile width on the real batch
+    and returns the fastest; the run with it matches the default plan."""
+    from tinyfluids_amd import engine
+    specs = specs_mixed()
+    init = engine.upload(specs, DEV)
+    tuned = engine.tune_batch(specs, init, candidates=(2, 4), steps=2)
+    assert tuned["tile_x"] in (2, 4)
+    assert set(tuned["candidates"]) == {"2,0", "4,0"}
+    assert all(v > 0 and math.isfinite(v) for v in tuned["candidates"].values())
+    assert tuned["ms_per_step"] == min(tuned["candidates"].values())
+    ref = engine.run_batched(specs, init, graph=False)
+    b = engine.run_batched(specs, init, graph=True, tile_x=tuned["tile_x"])
+    for x, y in zip(b.states, ref.states):
+        assert torch.equal(x, y)

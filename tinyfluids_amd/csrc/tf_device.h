@@ -123,16 +123,32 @@ __device__ __forceinline__ Q expand_qs(const QS &s) {
     return q;
 }
 
+// Coefficients of the HLL flux for Davis speeds S_R = sr >= 0 >= S_L = sl:
+// a = S_R/(S_R-S_L), b = S_L/(S_R-S_L), e = a S_L.
+struct HllCoef {
+    float a, b, e;
+};
+__device__ __forceinline__ HllCoef hll_coef(float sr, float sl) {
+    const float inv = frcp(sr - sl);
+    HllCoef c;
+    c.a = sr * inv;
+    c.b = sl * inv;
+    c.e = c.a * sl;
+    return c;
+}
+// One component: F = e (U_R - U_L) + (a F_L - b F_R), given dU = U_R - U_L
+// and aFL = a F_L. Every HLL flux of the step (x, y and z, halo and line
+// tilings) evaluates this same expression, so results do not depend on
+// where the left state came from (registers, a lane shift or LDS).
+__device__ __forceinline__ float hll_comp(const HllCoef &c, float dU, float aFL, float FR) {
+    return fmaf(c.e, dU, fmaf(-c.b, FR, aFL));
+}
+
 // HLL flux at the interface between L and R (fluid.py:261-305).
 __device__ __forceinline__ void hll(const Q &L, const Q &R, float F[5]) {
-    const float sr = fmax3(L.up, R.up, 0.0f);
-    const float sl = fmin3(L.um, R.um, 0.0f);
-    const float inv = frcp(sr - sl);
-    const float a = sr * inv;
-    const float b = sl * inv;
-    const float e = a * sl;
+    const HllCoef c = hll_coef(fmax3(L.up, R.up, 0.0f), fmin3(L.um, R.um, 0.0f));
 #pragma unroll
-    for (int i = 0; i < 5; ++i) F[i] = fmaf(e, R.U[i] - L.U[i], fmaf(a, L.F[i], -b * R.F[i]));
+    for (int i = 0; i < 5; ++i) F[i] = hll_comp(c, R.U[i] - L.U[i], c.a * L.F[i], R.F[i]);
 }
 
 // Godunov update U + k (F_left - F_right) (fluid.py:547-551).
@@ -164,7 +180,10 @@ __device__ __forceinline__ int cell_index(int i, int n, int open_bc) {
 
 // Lane shifts inside a wave64 (DPP wave_shr:1 / wave_shl:1): from_left gives
 // lane l the value of lane l-1, from_right the value of lane l+1; the edge
-// lane with no source reads 0 (bound_ctrl).
+// lane with no source reads 0 (bound_ctrl). Written right at their use, the
+// compiler folds them into the consuming VOP2 instruction (v_subrev_f32_dpp,
+// v_mul_f32_dpp) instead of issuing a separate v_mov_b32_dpp; it only looks
+// a few instructions ahead of the shift for that.
 __device__ __forceinline__ float from_left(float v) {
     return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x138, 0xf, 0xf, true));
 }

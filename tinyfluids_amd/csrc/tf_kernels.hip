@@ -76,6 +76,11 @@ __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t vof
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t field_rsrc(const float *field) {
     return __builtin_amdgcn_make_buffer_rsrc((void *)field, (short)0, -1, 0x00020000);
 }
+// An empty asm that takes v in a VGPR: v is computed before this point on
+// the lanes active here, and its computation cannot be sunk into a later
+// lane-masked block. Emits no instruction.
+__device__ __forceinline__ void pin_vgpr(float &v) { asm volatile("" : "+v"(v)); }
+
 __device__ __forceinline__ float sgpr_f(float v) {
     return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
 }
@@ -117,18 +122,17 @@ struct HaloCtx {
             for (int f = 0; f < 5; ++f) w[f] = buf_load(rin[f], zb, soff);
         }
     }
-    __device__ __forceinline__ void left(const Q &q, Q &ql, int) const {
+    // Z flux at the lower interface of this lane's cell (left state = lane - 1).
+    __device__ __forceinline__ void zflux(const Q &q, float (&F)[5], int) const {
+        const HllCoef c = hll_coef(fmax3(from_left(q.up), q.up, 0.0f), fmin3(from_left(q.um), q.um, 0.0f));
 #pragma unroll
-        for (int i = 0; i < 5; ++i) {
-            ql.U[i] = from_left(q.U[i]);
-            ql.F[i] = from_left(q.F[i]);
-        }
-        ql.up = from_left(q.up);
-        ql.um = from_left(q.um);
+        for (int i = 0; i < 5; ++i) F[i] = hll_comp(c, q.U[i] - from_left(q.U[i]), c.a * from_left(q.F[i]), q.F[i]);
     }
-    __device__ __forceinline__ void right(const float (&fz)[5], float (&fzr)[5], int) const {
+    // Godunov update with the upper interface's flux taken from lane + 1.
+    __device__ __forceinline__ void zupdate(const float (&U)[5], const float (&F)[5], float k, float (&out)[5],
+                                            int) const {
 #pragma unroll
-        for (int i = 0; i < 5; ++i) fzr[i] = from_right(fz[i]);
+        for (int i = 0; i < 5; ++i) out[i] = fmaf(k, F[i] - from_right(F[i]), U[i]);
     }
     __device__ __forceinline__ bool out_ok(int s) const { return zout && s < nxo; }
     __device__ __forceinline__ void store(int s, int y, const float (&w)[5]) const {
@@ -169,7 +173,8 @@ struct LineCtx {
 #pragma unroll
         for (int f = 0; f < 5; ++f) w[f] = buf_load(rin[f], xz[j], yoff);
     }
-    __device__ __forceinline__ void left(const Q &q, Q &ql, int s) const {
+    // Same arithmetic as HaloCtx::zflux / zupdate, neighbours through LDS.
+    __device__ __forceinline__ void zflux(const Q &q, float (&F)[5], int s) const {
         float *b = lq + (s & 1) * 12 * 64 * W;
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
@@ -179,21 +184,20 @@ struct LineCtx {
         b[10 * 64 * W + gl] = q.up;
         b[11 * 64 * W + gl] = q.um;
         __syncthreads();
+        const HllCoef c =
+            hll_coef(fmax3(b[10 * 64 * W + lidx], q.up, 0.0f), fmin3(b[11 * 64 * W + lidx], q.um, 0.0f));
 #pragma unroll
-        for (int i = 0; i < 5; ++i) {
-            ql.U[i] = b[i * 64 * W + lidx];
-            ql.F[i] = b[(5 + i) * 64 * W + lidx];
-        }
-        ql.up = b[10 * 64 * W + lidx];
-        ql.um = b[11 * 64 * W + lidx];
+        for (int i = 0; i < 5; ++i)
+            F[i] = hll_comp(c, q.U[i] - b[i * 64 * W + lidx], c.a * b[(5 + i) * 64 * W + lidx], q.F[i]);
     }
-    __device__ __forceinline__ void right(const float (&fz)[5], float (&fzr)[5], int s) const {
+    __device__ __forceinline__ void zupdate(const float (&U)[5], const float (&F)[5], float k, float (&out)[5],
+                                            int s) const {
         float *b = lf + (s & 1) * 5 * 64 * W;
 #pragma unroll
-        for (int i = 0; i < 5; ++i) b[i * 64 * W + gl] = fz[i];
+        for (int i = 0; i < 5; ++i) b[i * 64 * W + gl] = F[i];
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < 5; ++i) fzr[i] = b[i * 64 * W + ridx];
+        for (int i = 0; i < 5; ++i) out[i] = fmaf(k, F[i] - b[i * 64 * W + ridx], U[i]);
     }
     __device__ __forceinline__ bool out_ok(int s) const { return valid && xs0 + s < nx; }
     __device__ __forceinline__ void store(int s, int y, const float (&w)[5]) const {
@@ -244,15 +248,16 @@ __device__ __forceinline__ void stream_row(const Ctx &T, const Gas &g, int r, fl
             godunov(qa[s].U, fa[s], fb[s], g.k, U2);  // the previous row is now Y-swept
             const Prim w2 = cons_to_prim(U2, g);
             const Q q2 = make_q<3>(U2, w2.v, w2.p, w2.rinv, g);
-            Q ql;
-            T.left(q2, ql, s);
-            float fz[5], fzr[5];
-            hll(ql, q2, fz);
-            T.right(fz, fzr, s);
-            float U3[5];
-            godunov(q2.U, fz, fzr, g.k, U3);
-            const Prim w3 = cons_to_prim(U3, g);
+            float fz[5], U3[5];
+            T.zflux(q2, fz, s);
+            T.zupdate(q2.U, fz, g.k, U3, s);
+            // U3 is complete here on every lane: no exec-mask change separates
+            // the lane shifts above from their uses (which lets them fold into
+            // the arithmetic); only the output lanes continue below.
+#pragma unroll
+            for (int i = 0; i < 5; ++i) pin_vgpr(U3[i]);
             if (T.out_ok(s)) {
+                const Prim w3 = cons_to_prim(U3, g);
                 const float wo[5] = {w3.r, w3.v[0], w3.v[1], w3.v[2], w3.p};
                 T.store(s, T.y0 + r - 2, wo);
                 const float sp = signal_speed(w3.v, sound(w3.p, w3.rinv, g));

@@ -53,6 +53,28 @@ class BatchResult:
     waves: int = 0
 
 
+def tune_batch(specs: Sequence[SimSpec], initial: Sequence[torch.Tensor], candidates=(2, 4, 6, 8), steps: int = 4,
+               tile_y_candidates=(0,)) -> Dict[str, float]:
+    """Times `steps` batched steps of the real batch for each tile width
+    (and y-chunk) candidate on the device (HIP events around the stencil
+    launches) and returns the fastest as {"tile_x", "tile_y", "ms_per_step"}
+    plus every candidate's time."""
+    device = initial[0].device
+    sims = [Simulation(x, s.dx, s.gamma, device=device) for s, x in zip(specs, initial)]
+    results = {}
+    for tx in candidates:
+        for ty in tile_y_candidates:
+            b = Batch(sims, tile_x=tx, tile_y=ty)
+            for s in sims:
+                s.reset()
+            b.step(1)  # warm-up
+            results[(tx, ty)] = b.step_timed(steps) / steps
+            b.close()
+    best = min(results, key=results.get)
+    return {"tile_x": best[0], "tile_y": best[1], "ms_per_step": results[best],
+            "candidates": {f"{k[0]},{k[1]}": v for k, v in results.items()}}
+
+
 def run_batched(specs: Sequence[SimSpec], initial: Sequence[torch.Tensor], *, graph: bool = True, tile_x: int = 0,
                 tile_y: int = 0, keep_states: bool = True) -> BatchResult:
     """All simulations advanced together for max(spec.steps) batched steps
