@@ -41,6 +41,9 @@ def parse():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=40)
     ap.add_argument("--serve-rates", type=str, default="8,32", help="Poisson arrival rates (jobs/s) to sweep")
+    ap.add_argument("--no-harness", action="store_true")
+    ap.add_argument("--harness-sizes", type=str, default="32,64,128,256,384,512,768")
+    ap.add_argument("--no-shapes", action="store_true", help="skip the non-cubic 1536x512x512 roofline line")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also copy the state buffer once (a plain copy kernel of known size, for HBM counters)")
     return ap.parse_args()
@@ -77,6 +80,73 @@ def cpu_baseline(nx, ny, budget_s):
                   f"z-slab, {steps} steps in {el:.1f} s, OpenMP over all host threads; "
                   f"host {os.cpu_count()} logical CPUs",
     }
+
+
+def cpu_numpy_64():
+    """The NumPy restatement of the reference step (oracle/np_tinyfluids.py)
+    on the 64^3 spherical blast (gamma 5/3, CFL 0.4), float32 (the reference's
+    precision) and float64 (the oracle): 1 warm-up step, median of 3 timed
+    steps. NumPy's elementwise work runs on one core in effect."""
+    import numpy as np
+
+    from oracle import np_tinyfluids as npo
+    from tinyfluids_amd import ics
+
+    out = {}
+    s0, dx = ics.blast((64, 64, 64))
+    for name, dt in (("f32", np.float32), ("f64", np.float64)):
+        s = s0.astype(dt)
+        s = npo.run_steps(s, dx, ics.GAMMA_BLAST, 1)[0]
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            s = npo.run_steps(s, dx, ics.GAMMA_BLAST, 1)[0]
+            times.append(time.perf_counter() - t0)
+        out[name] = 64 ** 3 / sorted(times)[1]
+    return {"value_f32": out["f32"], "value_f64": out["f64"], "unit": "cell-updates/s", "cores": 1,
+            "kind": "port", "sample": "NumPy restatement of the reference step, 64^3 blast, median of 3 steps"}
+
+
+# Wall time of time_integration to t = 0.2 on setup_ics(N, N // 16) on one
+# NVIDIA GH200, from the reference's own log (scaling_jax_57440.out:6-14).
+GH200_SECONDS = {32: 0.001980, 64: 0.006428, 96: 0.024395, 128: 0.082908, 196: 0.559784, 256: 1.464719,
+                 384: 7.514550, 512: 24.537891, 768: 128.716188}
+
+
+def harness_leg(dev, sizes):
+    """The reference's own benchmark (scaling_jax.py:42-73, 86-93): the blast
+    cube setup_ics(N, N // 16), gamma 5/3, dx 1/(N-1), time_integration to
+    t = 0.2 without open boundaries; one warm-up call, then the minimum of 3
+    timed calls, each returning after the run (block_until_ready)."""
+    import torch
+
+    import tinyfluids_amd as tf
+    from tinyfluids_amd import ics
+
+    out = []
+    for n in sizes:
+        s, dx = ics.setup_ics(n, n // 16)
+        x = torch.from_numpy(s).to(dev)
+        del s
+        _, iters = tf.time_integration(x, dx, 0.2, 5 / 3, False, False)
+        times = []
+        for _ in range(3):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            y, it = tf.time_integration(x, dx, 0.2, 5 / 3, False, False)
+            times.append(time.perf_counter() - t0)
+            assert it == iters
+            del y
+        t = min(times)
+        row = {"N": n, "iterations": iters, "seconds": t, "cell_updates_per_s": n ** 3 * iters / t}
+        if n in GH200_SECONDS:
+            row["gh200_seconds"] = GH200_SECONDS[n]
+            row["speedup_vs_gh200"] = GH200_SECONDS[n] / t
+        out.append(row)
+        del x
+        torch.cuda.empty_cache()
+        print(f"bench: harness {row}", file=sys.stderr, flush=True)
+    return out
 
 
 def measured_traffic(nx, ny, nz, tile):
@@ -134,7 +204,7 @@ def serve_leg(dev, njobs, rates):
     out = []
     for rate in rates:
         specs = ics.serving_stream(njobs, rate, seed=4)
-        srv = engine.Server(hbm_budget_bytes=64 << 30, steps_per_tick=16)
+        srv = engine.Server(hbm_budget_bytes=64 << 30)
         r = srv.serve(specs).summary()
         p = engine.serve_plain(specs).summary()
         out.append({"rate_jobs_per_s": rate, "jobs": njobs,
@@ -147,6 +217,63 @@ def serve_leg(dev, njobs, rates):
     return out
 
 
+def measure_grid(dev, nx, ny, nz, steps, warmup, tile, no_tune, calibrate):
+    """KH on nx x ny x nz resident in HBM: tiling (forced, autotuned or the
+    default), `warmup` steps, then `steps` timed steps (wall clock and HIP
+    events around the stencil launches)."""
+    import ctypes
+
+    import torch
+
+    import tinyfluids_amd as tf
+    from tinyfluids_amd import _lib, ics
+    from tinyfluids_amd.batch import Batch, Simulation
+
+    dx = 1.0 / ny
+    gamma = ics.GAMMA_KH
+    plane = torch.from_numpy(ics.kelvin_helmholtz_plane(nx, ny, dx)).to(dev)
+    buf0 = plane[:, :, :, None].expand(5, nx, ny, nz).contiguous()
+    del plane
+    buf1 = torch.empty_like(buf0)
+    print(f"bench: KH {nx}x{ny}x{nz} fp32, state {buf0.numel() * 4 / 1e9:.2f} GB", file=sys.stderr, flush=True)
+    lib = _lib.load()
+    tx, ly = ctypes.c_int(0), ctypes.c_int(0)
+    tuned_ms = None
+    if tile:
+        tx.value, ly.value = (int(v) for v in tile.split(","))
+        _lib.check(lib.tf_set_tiling(nx, ny, nz, tx.value, ly.value))
+    elif not no_tune:
+        ws = tf.fluid.workspace(nx, ny, nz, dev)
+        ms = ctypes.c_float(0)
+        _lib.check(lib.tf_autotune_evolve(buf0.data_ptr(), buf1.data_ptr(), nx, ny, nz, dx, gamma, ws.data_ptr(),
+                                          ctypes.byref(tx), ctypes.byref(ly), ctypes.byref(ms),
+                                          tf.fluid._stream(dev)))
+        tuned_ms = ms.value
+        del ws
+    else:
+        _lib.check(lib.tf_get_tiling(nx, ny, nz, ctypes.byref(tx), ctypes.byref(ly)))
+    print(f"bench: tiling tile_x={tx.value} tile_y={ly.value} (tuned {tuned_ms} ms)", file=sys.stderr, flush=True)
+    if calibrate:
+        buf1.copy_(buf0)
+        torch.cuda.synchronize()
+        print(f"bench: calibration copy of {buf0.numel() * 4} bytes", file=sys.stderr, flush=True)
+    sim = Simulation(buf0, dx, gamma, buffers=(buf0, buf1))
+    batch = Batch([sim], tile_x=tx.value, tile_y=ly.value)
+    sim.reset()
+    batch.step(warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stencil_ms = batch.step_timed(steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    st = batch.status()[0]
+    assert st.n_iter == warmup + steps and not st.nonfinite, (st.n_iter, st.nonfinite)
+    batch.close()
+    del buf0, buf1, sim, batch
+    torch.cuda.empty_cache()
+    return {"tile_x": tx.value, "tile_y": ly.value, "status": st, "seconds": el, "stencil_ms": stencil_ms}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -155,58 +282,13 @@ def main():
         sys.exit(2)
     import torch
 
-    import tinyfluids_amd as tf
-    from tinyfluids_amd import ics
-    from tinyfluids_amd.batch import Batch, Simulation
-    from tinyfluids_amd import _lib
-
     nx, ny, nz = (int(v) for v in args.shape.split(","))
     dev = torch.device("cuda:0")
-    dx = 1.0 / ny
-    gamma = ics.GAMMA_KH
-    plane = torch.from_numpy(ics.kelvin_helmholtz_plane(nx, ny, dx)).to(dev)
-    buf0 = plane[:, :, :, None].expand(5, nx, ny, nz).contiguous()
-    del plane
-    buf1 = torch.empty_like(buf0)
-    print(f"bench: KH {nx}x{ny}x{nz} fp32, state {buf0.numel() * 4 / 1e9:.2f} GB", file=sys.stderr, flush=True)
-
-    lib = _lib.load()
-    import ctypes
-    tx, ly = ctypes.c_int(0), ctypes.c_int(0)
-    tuned_ms = None
-    if args.tile:
-        tx.value, ly.value = (int(v) for v in args.tile.split(","))
-        _lib.check(lib.tf_set_tiling(nx, ny, nz, tx.value, ly.value))
-    elif not args.no_tune:
-        ws = tf.fluid.workspace(nx, ny, nz, dev)
-        ms = ctypes.c_float(0)
-        _lib.check(lib.tf_autotune_evolve(buf0.data_ptr(), buf1.data_ptr(), nx, ny, nz, dx, gamma, ws.data_ptr(),
-                                          ctypes.byref(tx), ctypes.byref(ly), ctypes.byref(ms),
-                                          tf.fluid._stream(dev)))
-        tuned_ms = ms.value
-    else:
-        _lib.check(lib.tf_get_tiling(nx, ny, nz, ctypes.byref(tx), ctypes.byref(ly)))
-    print(f"bench: tiling tile_x={tx.value} tile_y={ly.value} (tuned {tuned_ms} ms)", file=sys.stderr, flush=True)
-
-    if args.calibrate_copy:
-        buf1.copy_(buf0)
-        torch.cuda.synchronize()
-        print(f"bench: calibration copy of {buf0.numel() * 4} bytes", file=sys.stderr, flush=True)
-    sim = Simulation(buf0, dx, gamma, buffers=(buf0, buf1))
-    batch = Batch([sim], tile_x=tx.value, tile_y=ly.value)
-    sim.reset()
-    batch.step(args.warmup)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    stencil_ms = batch.step_timed(args.steps)
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
-    st = batch.status()[0]
-    assert st.n_iter == args.warmup + args.steps and not st.nonfinite, (st.n_iter, st.nonfinite)
-
+    m = measure_grid(dev, nx, ny, nz, args.steps, args.warmup, args.tile, args.no_tune, args.calibrate_copy)
+    tx, ly, st, el, stencil_ms = m["tile_x"], m["tile_y"], m["status"], m["seconds"], m["stencil_ms"]
     cells = nx * ny * nz
     value = cells * args.steps / el
-    traffic, traffic_src = measured_traffic(nx, ny, nz, (tx.value, ly.value))
+    traffic, traffic_src = measured_traffic(nx, ny, nz, (tx, ly))
     kernel_s = stencil_ms / 1e3 / args.steps
     achieved = BYTES_PER_CELL * cells / kernel_s / 1e9
     rec = {
@@ -224,15 +306,25 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"Kelvin-Helmholtz shear layer {nx}x{ny}x{nz} periodic, gamma 1.4, dx 1/{ny}, "
                                f"CFL 0.4, fused split HLL/Godunov step with dt",
-                   "grid": [nx, ny, nz], "tile_x": tx.value, "tile_y": ly.value, "parallelism": "single GPU"},
+                   "grid": [nx, ny, nz], "tile_x": tx, "tile_y": ly, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
-                     "kernel": f"tf::k_step_sims<{tx.value}>", "kernel_ms": kernel_s * 1e3,
+                     "kernel": f"tf::k_step_sims<{tx}>", "kernel_ms": kernel_s * 1e3,
                      "algorithmic_bytes_per_launch": BYTES_PER_CELL * cells},
         "sim_time": st.t,
     }
-    del buf0, buf1, sim, batch
+    if not args.no_shapes and (nx, ny, nz) == (1024, 1024, 1024):
+        m2 = measure_grid(dev, 1536, 512, 512, 20, 2, "", args.no_tune, False)
+        c2 = 1536 * 512 * 512
+        k2 = m2["stencil_ms"] / 1e3 / 20
+        rec["shapes"] = [{"grid": [1536, 512, 512], "workload": "Kelvin-Helmholtz, gamma 1.4, dx 1/512, 20 steps",
+                          "value": c2 * 20 / m2["seconds"], "ms_per_step": m2["seconds"] * 1e3 / 20,
+                          "kernel_ms": k2 * 1e3, "frac": BYTES_PER_CELL * c2 / k2 / 1e9 / HBM_PEAK_GBS,
+                          "tile_x": m2["tile_x"], "tile_y": m2["tile_y"]}]
+        print(f"bench: shapes {rec['shapes']}", file=sys.stderr, flush=True)
     torch.cuda.empty_cache()
+    if not args.no_harness:
+        rec["reference_harness"] = harness_leg(dev, [int(v) for v in args.harness_sizes.split(",")])
     if not args.no_batched:
         rec["batched"] = batched_leg(dev)
         print(f"bench: batched {rec['batched']}", file=sys.stderr, flush=True)
@@ -242,6 +334,7 @@ def main():
         torch.cuda.empty_cache()
     if not args.no_cpu:
         rec["cpu_baseline"] = cpu_baseline(nx, ny, args.cpu_seconds)
+        rec["cpu_numpy_64"] = cpu_numpy_64()
     print(json.dumps(rec), flush=True)
 
 

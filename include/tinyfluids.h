@@ -187,7 +187,8 @@ int tf_batch_status(tf_batch *b, tf_sim_status *host_out, void *stream);
  * Tiling and autotuning of the fused stencil. No reference counterpart.
  * ------------------------------------------------------------------------ */
 
-/* Tiling used for an nx*ny*nz grid by tf_evolve_state / tf_time_integration. */
+/* Tiling used for an nx*ny*nz grid by tf_evolve_state; tf_set_tiling also
+ * fixes it for tf_time_integration. */
 int tf_get_tiling(int nx, int ny, int nz, int *tile_x, int *tile_y);
 int tf_set_tiling(int nx, int ny, int nz, int tile_x, int tile_y);
 
@@ -195,6 +196,15 @@ int tf_set_tiling(int nx, int ny, int nz, int tile_x, int tile_y);
  * fastest for this shape (tf_set_tiling) and reports it. Synchronous. */
 int tf_autotune_evolve(const float *in, float *out, int nx, int ny, int nz, double grid_spacing, double gamma,
                        void *workspace, int *tile_x, int *tile_y, float *ms_per_step, void *stream);
+
+/* Times the candidate tilings of the step that tf_time_integration runs
+ * (tile width 2/4/8 x y-chunk rows, halo or whole-z-line tiling) on this
+ * state, keeps the fastest for this shape and reports it (tile_y as
+ * resolved). Reads state, writes scratch; synchronous. No reference
+ * counterpart (JAX compiles per shape on the first call instead). */
+int tf_autotune_loop(float *state, float *scratch, int nx, int ny, int nz, double grid_spacing, double gamma,
+                     int handle_boundaries, void *workspace, int *tile_x, int *tile_y, float *ms_per_step,
+                     void *stream);
 
 #ifdef __cplusplus
 }

@@ -152,3 +152,27 @@ def test_nan_ends_loop_like_reference():
     assert num == 1
     with pytest.raises(FloatingPointError):
         tf().time_integration(to_dev(s), dx, 0.2, 5 / 3, False, False, check_finite=True)
+
+
+@pytest.mark.parametrize("shape,hb", [((13, 36, 40), False), ((9, 21, 70), True)])
+def test_autotune_loop_is_read_only_and_tiling_free(shape, hb):
+    """The loop tuner reads the state and writes only scratch; runs under the
+    tuned tiling, the untuned default and forced tilings agree bit for bit,
+    and a time_integration run ends on the oracle's step count."""
+    import tinyfluids_amd._lib as L
+    from tinyfluids_amd.time_integration import autotune, run_steps
+    s = to_dev(random_state(shape, 31))
+    keep = s.clone()
+    tx, ty, ms = autotune(s, 0.05, 1.4, hb)
+    assert torch.equal(s, keep)
+    assert tx in (2, 4, 8) and 1 <= ty <= shape[1] and ms > 0
+    ref, t_ref, _ = run_steps(s, 0.05, 1.4, 7, handle_boundaries=hb)
+    for ftx, fty in ((2, 1), (4, 3), (8, shape[1])):
+        L.check(L.load().tf_set_tiling(*shape, ftx, fty))
+        out, t, _ = run_steps(s, 0.05, 1.4, 7, handle_boundaries=hb)
+        assert torch.equal(out, ref), (ftx, fty)
+        assert t == t_ref
+    s64 = keep.cpu().numpy().astype(np.float64)
+    _, n_ref = npo.time_integration(s64, 0.05, 0.02, 1.4, hb)
+    _, n = tf().time_integration(s, 0.05, 0.02, 1.4, hb, False)
+    assert n == n_ref

@@ -79,6 +79,8 @@ SIGNATURES = {
     "tf_set_tiling": (I, [I, I, I, I, I]),
     "tf_autotune_evolve": (I, [P, P, I, I, I, D, D, P, ctypes.POINTER(I), ctypes.POINTER(I),
                                ctypes.POINTER(ctypes.c_float), P]),
+    "tf_autotune_loop": (I, [P, P, I, I, I, D, D, I, P, ctypes.POINTER(I), ctypes.POINTER(I),
+                             ctypes.POINTER(ctypes.c_float), P]),
 }
 
 

@@ -110,6 +110,39 @@ int default_tile_y(int nx, int ny, int nz, int tx) {
     return std::max(1, ly);
 }
 
+// Tiling of the one-simulation plan that tf_time_integration runs (set by
+// tf_autotune_loop or tf_set_tiling); tile_y 0 lets the plan choose.
+std::map<std::tuple<int, int, int>, std::pair<int, int>> g_loop_tiling;
+
+// Untuned loop tiling: narrow tiles for grids too small to fill the chip,
+// so that each wave's serial stream stays short.
+int default_loop_tx(int nx, int ny, int nz) {
+    const double cells = (double)nx * ny * nz;
+    if (cells >= 16777216.0) return default_tile_x(nx);
+    if (cells >= 2097152.0) return nx >= 4 ? 4 : 2;
+    return 2;
+}
+
+void loop_tiling_for(int nx, int ny, int nz, int &tx, int &ly) {
+    {
+        std::lock_guard<std::mutex> lk(g_tiling_mu);
+        auto it = g_loop_tiling.find({nx, ny, nz});
+        if (it != g_loop_tiling.end()) {
+            tx = it->second.first;
+            ly = it->second.second;
+            return;
+        }
+        auto jt = g_tiling.find({nx, ny, nz});
+        if (jt != g_tiling.end()) {
+            tx = jt->second.first;
+            ly = jt->second.second;
+            return;
+        }
+    }
+    tx = default_loop_tx(nx, ny, nz);
+    ly = 0;
+}
+
 void tiling_for(int nx, int ny, int nz, int &tx, int &ly) {
     {
         std::lock_guard<std::mutex> lk(g_tiling_mu);
@@ -189,11 +222,16 @@ int tf_abi_version(void) { return TF_ABI_VERSION; }
 
 const char *tf_last_error(void) { return g_err.c_str(); }
 
+// Workspace of the single-grid entry points: the header, the float64 totals
+// partials, then the one-simulation plan of tf_time_integration.
+constexpr size_t kLoopPlanOffset = kWsHeaderBytes + (size_t)kTotalsBlocks * 5 * sizeof(double);
+static_assert(kLoopPlanOffset % 256 == 0, "plan workspace alignment");
+
 size_t tf_workspace_bytes(int nx, int ny, int nz) {
     (void)nx;
     (void)ny;
     (void)nz;
-    return kWsHeaderBytes + (size_t)kTotalsBlocks * 5 * sizeof(double);
+    return kLoopPlanOffset + tf_batch_workspace_bytes(1);
 }
 
 int tf_conserved_state_from_primitive(const float *prim, float *cons, int64_t ncells, double gamma, void *stream) {
@@ -261,53 +299,6 @@ int tf_evolve_state(const float *in, float *out, int nx, int ny, int nz, double 
     return TF_OK;
 }
 
-int tf_time_integration(float *state, float *scratch, int nx, int ny, int nz, double grid_spacing, double t_final,
-                        double gamma, int handle_boundaries, int64_t max_steps, double c_cfl, void *workspace,
-                        int64_t *num_iterations, double *t_out, int *result_in_scratch, int *nonfinite,
-                        void *stream) {
-    if (int e = check_dims(nx, ny, nz)) return e;
-    if (int e = check_fused_size(nx, ny, nz)) return e;
-    TF_REQUIRE(state && scratch && workspace, "null device pointer");
-    TF_REQUIRE(num_iterations && t_out && result_in_scratch, "null host result pointer");
-    TF_REQUIRE(state != scratch, "state and scratch must be different buffers");
-    if (int e = check_open_bc(handle_boundaries, nx, ny, nz)) return e;
-    hipStream_t st = S_(stream);
-    auto *H = reinterpret_cast<WsHeader *>(workspace);
-    int tx, ly;
-    tiling_for(nx, ny, nz, tx, ly);
-    SimDev S = make_simdev(nx, ny, nz, handle_boundaries, gamma, tx, ly);
-    S.buf[0] = state;
-    S.buf[1] = scratch;
-    S.ctrl = &H->ctrl;
-    const int prefix[2] = {0, S.waves};
-    const FarList F = far_list(S, tx);
-    TF_HIP(hipMemcpyAsync(&H->sim, &S, sizeof S, hipMemcpyHostToDevice, st));
-    TF_HIP(hipMemcpyAsync(H->prefix, prefix, sizeof prefix, hipMemcpyHostToDevice, st));
-    TF_HIP(hipMemcpyAsync(&H->far, &F, sizeof F, hipMemcpyHostToDevice, st));
-    TF_HIP(launch_ctrl_reset(&H->ctrl, grid_spacing, c_cfl, t_final, max_steps, st));
-    TF_HIP(launch_speed(state, (int64_t)nx * ny * nz, (float)gamma, &H->ctrl.max_bits[0], st));
-    TF_HIP(launch_finalize_one(&H->ctrl, 1, st));
-    // Steps run in chunks; a step whose control block says inactive returns
-    // at once, so the host checks the loop condition once per chunk.
-    Ctrl host;
-    int chunk = 4;
-    for (;;) {
-        for (int i = 0; i < chunk; ++i) {
-            TF_HIP(launch_step_sims(tx, &H->sim, H->prefix, 1, S.waves, st));
-            TF_HIP(launch_step_sims_far(tx, &H->sim, H->far.t, F.n, S.ntz, st));
-            TF_HIP(launch_finalize_sims(&H->sim, 1, st));
-        }
-        TF_HIP(hipMemcpyAsync(&host, &H->ctrl, sizeof host, hipMemcpyDeviceToHost, st));
-        TF_HIP(hipStreamSynchronize(st));
-        if (!host.active) break;
-        chunk = std::min(chunk * 2, 64);
-    }
-    *num_iterations = host.n_iter;
-    *t_out = host.t;
-    *result_in_scratch = (int)(host.n_iter & 1);
-    if (nonfinite) *nonfinite = host.nonfinite;
-    return TF_OK;
-}
 
 int tf_conserved_totals(const float *prim, int nx, int ny, int nz, double gamma, double *out5, void *workspace,
                         void *stream) {
@@ -332,6 +323,7 @@ int tf_set_tiling(int nx, int ny, int nz, int tile_x, int tile_y) {
     TF_REQUIRE(tile_y >= 1, "tile_y must be >= 1");
     std::lock_guard<std::mutex> lk(g_tiling_mu);
     g_tiling[{nx, ny, nz}] = {tile_x, std::min(tile_y, ny)};
+    g_loop_tiling[{nx, ny, nz}] = {tile_x, std::min(tile_y, ny)};
     return TF_OK;
 }
 
@@ -480,7 +472,34 @@ int tf_batch_create(tf_batch **out, const tf_sim_desc *sims, int nsims, void *wo
     TF_REQUIRE(sims && workspace, "null pointer");
     TF_REQUIRE(workspace_bytes >= tf_batch_workspace_bytes(nsims), "workspace too small: %zu < %zu",
                workspace_bytes, tf_batch_workspace_bytes(nsims));
-    const int tx = tile_x == 0 ? 8 : tile_x;
+    double total_cells = 0.0, max_cells = 0.0;
+    int largest = 0;
+    for (int i = 0; i < nsims; ++i) {
+        const double c = (double)sims[i].nx * sims[i].ny * sims[i].nz;
+        total_cells += c;
+        if (c > max_cells) {
+            max_cells = c;
+            largest = i;
+        }
+    }
+    // Measured loop tilings (tf_autotune_loop / tf_set_tiling) of the
+    // simulations' shapes, used when neither tile_x nor tile_y is given: the
+    // tile width of the largest simulation and each shape's own y-chunks.
+    std::vector<int> tuned_ly(nsims, 0);
+    int tuned_tx = 0;
+    if (tile_x == 0 && tile_y == 0) {
+        std::lock_guard<std::mutex> lk(g_tiling_mu);
+        for (int i = 0; i < nsims; ++i) {
+            auto it = g_loop_tiling.find({sims[i].nx, sims[i].ny, sims[i].nz});
+            if (it == g_loop_tiling.end()) continue;
+            tuned_ly[i] = it->second.second;
+            if (i == largest) tuned_tx = it->second.first;
+        }
+    }
+    // Default tile width: 8 for work that saturates HBM (fewest halo
+    // columns); 4 below that, where more resident waves per SIMD hide the
+    // latency of many short streams.
+    const int tx = tile_x != 0 ? tile_x : (tuned_tx != 0 ? tuned_tx : (total_cells >= 268435456.0 ? 8 : 4));
     TF_REQUIRE(valid_tx(tx), "tile_x must be 0, 2, 4, 6 or 8 (got %d)", tile_x);
     TF_REQUIRE(tile_y >= 0, "tile_y must be >= 0");
     for (int i = 0; i < nsims; ++i) {
@@ -495,7 +514,8 @@ int tf_batch_create(tf_batch **out, const tf_sim_desc *sims, int nsims, void *wo
         long total = 0;
         for (int i = 0; i < nsims; ++i) {
             const tf_sim_desc &d = sims[i];
-            const int ly = tile_y > 0 ? tile_y : std::max(1, std::min(d.ny, ly_cap));
+            const int ly = tile_y > 0 ? tile_y
+                                      : (tuned_ly[i] > 0 ? tuned_ly[i] : std::max(1, std::min(d.ny, ly_cap)));
             SimDev S = make_simdev(d.nx, d.ny, d.nz, d.handle_boundaries, d.gamma, tx, ly);
             S.buf[0] = d.buf0;
             S.buf[1] = d.buf1;
@@ -514,18 +534,30 @@ int tf_batch_create(tf_batch **out, const tf_sim_desc *sims, int nsims, void *wo
         }
         return total;
     };
+    // A batched step lasts at least as long as its longest wave's serial
+    // stream (rows x tile width x ~0.37 us per column-row at gfx950 clocks),
+    // and at best as long as its HBM traffic (40 B per cell at ~4.5 TB/s).
+    // Without a given tile_y, y-chunks are capped so that the longest stream
+    // stays within half the traffic time.
+    int ly_cap = 1 << 30;
+    if (tile_y == 0) {
+        const double step_us = total_cells * 40.0 / 4.5e6;
+        const double row_us = tx * 0.37;
+        ly_cap = std::max(4, (int)(0.5 * step_us / row_us) - 2);
+    }
     std::vector<SimDev> tab(nsims);
-    long total = build(1 << 30, tab);
+    long total = build(ly_cap, tab);
     if (tile_y == 0) {
         // Whole y-lines per wave unless the batch is too small to fill the
         // chip several times over; then halve every simulation's y-chunks
-        // (down to 16 rows).
+        // (down to 4 rows: a small batch is bound by the length of each
+        // wave's serial stream, not by the halo rows).
         const long target = 256L * 4 * 3 * 6;
-        for (int cap = 1 << 30; total < target && cap > 16;) {
+        for (int cap = ly_cap; total < target && cap > 4;) {
             int maxny = 1;
             for (int i = 0; i < nsims; ++i) maxny = std::max(maxny, sims[i].ny);
             cap = std::min(cap, maxny) / 2;
-            if (cap < 16) cap = 16;
+            if (cap < 4) cap = 4;
             total = build(cap, tab);
         }
     }
@@ -710,3 +742,151 @@ int tf_batch_status(tf_batch *b, tf_sim_status *host_out, void *stream) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ single-grid loop
+namespace {
+struct PlanGuard {
+    tf_batch *b = nullptr;
+    ~PlanGuard() { tf_batch_destroy(b); }
+};
+
+int loop_args(float *state, float *scratch, int nx, int ny, int nz, int handle_boundaries, void *workspace) {
+    if (int e = check_dims(nx, ny, nz)) return e;
+    if (int e = check_fused_size(nx, ny, nz)) return e;
+    TF_REQUIRE(state && scratch && workspace, "null device pointer");
+    TF_REQUIRE(state != scratch, "state and scratch must be different buffers");
+    if (int e = check_open_bc(handle_boundaries, nx, ny, nz)) return e;
+    return TF_OK;
+}
+
+tf_sim_desc loop_desc(float *state, float *scratch, int nx, int ny, int nz, double grid_spacing, double t_final,
+                      double gamma, int handle_boundaries, int64_t max_steps, double c_cfl, void *workspace) {
+    tf_sim_desc d;
+    std::memset(&d, 0, sizeof d);
+    d.buf0 = state;
+    d.buf1 = scratch;
+    d.ctrl = &reinterpret_cast<WsHeader *>(workspace)->ctrl;
+    d.nx = nx;
+    d.ny = ny;
+    d.nz = nz;
+    d.handle_boundaries = handle_boundaries;
+    d.grid_spacing = grid_spacing;
+    d.gamma = gamma;
+    d.c_cfl = c_cfl;
+    d.t_final = t_final;
+    d.max_steps = max_steps;
+    return d;
+}
+
+void *plan_workspace(void *workspace) { return static_cast<char *>(workspace) + kLoopPlanOffset; }
+}  // namespace
+
+int tf_time_integration(float *state, float *scratch, int nx, int ny, int nz, double grid_spacing, double t_final,
+                        double gamma, int handle_boundaries, int64_t max_steps, double c_cfl, void *workspace,
+                        int64_t *num_iterations, double *t_out, int *result_in_scratch, int *nonfinite,
+                        void *stream) {
+    if (int e = loop_args(state, scratch, nx, ny, nz, handle_boundaries, workspace)) return e;
+    TF_REQUIRE(num_iterations && t_out && result_in_scratch, "null host result pointer");
+    hipStream_t st = S_(stream);
+    const tf_sim_desc d = loop_desc(state, scratch, nx, ny, nz, grid_spacing, t_final, gamma, handle_boundaries,
+                                    max_steps, c_cfl, workspace);
+    int tx, ly;
+    loop_tiling_for(nx, ny, nz, tx, ly);
+    PlanGuard plan;
+    if (int e = tf_batch_create(&plan.b, &d, 1, plan_workspace(workspace), tf_batch_workspace_bytes(1), tx, ly,
+                                stream))
+        return e;
+    if (int e = tf_sim_reset(&d, stream)) return e;
+    // Steps run in chunks; a step whose control block says inactive returns
+    // at once. The host reads the control block after each chunk and sizes
+    // the next one from the remaining steps or the remaining time at the
+    // current dt, so a run needs few synchronisations.
+    const Ctrl *C = reinterpret_cast<const Ctrl *>(d.ctrl);
+    Ctrl host;
+    int64_t chunk = max_steps >= 0 && std::isinf(t_final) ? max_steps : 4;
+    for (;;) {
+        chunk = std::max<int64_t>(1, std::min<int64_t>(chunk, 4096));
+        if (max_steps != 0)
+            if (int e = enqueue_steps(plan.b, (int)chunk, st)) return e;
+        TF_HIP(hipMemcpyAsync(&host, C, sizeof host, hipMemcpyDeviceToHost, st));
+        TF_HIP(hipStreamSynchronize(st));
+        if (!host.active) break;
+        int64_t est = max_steps >= 0 ? max_steps - host.n_iter : (int64_t)1 << 40;
+        if (std::isfinite(t_final) && host.dt > 0.0)
+            est = std::min<int64_t>(est, (int64_t)std::ceil(std::min((t_final - host.t) / host.dt, 1e9)) + 1);
+        chunk = est;
+    }
+    *num_iterations = host.n_iter;
+    *t_out = host.t;
+    *result_in_scratch = (int)(host.n_iter & 1);
+    if (nonfinite) *nonfinite = host.nonfinite;
+    return TF_OK;
+}
+
+int tf_autotune_loop(float *state, float *scratch, int nx, int ny, int nz, double grid_spacing, double gamma,
+                     int handle_boundaries, void *workspace, int *tile_x, int *tile_y, float *ms_per_step,
+                     void *stream) {
+    if (int e = loop_args(state, scratch, nx, ny, nz, handle_boundaries, workspace)) return e;
+    TF_REQUIRE(tile_x && tile_y, "null pointer");
+    hipStream_t st = S_(stream);
+    const tf_sim_desc d = loop_desc(state, scratch, nx, ny, nz, grid_spacing, INFINITY, gamma, handle_boundaries,
+                                    -1, 0.4, workspace);
+    // dt from the state; the control block stays at step 0 (no dt update is
+    // launched), so every timed launch reads state and writes scratch.
+    if (int e = tf_sim_reset(&d, stream)) return e;
+    const double cells = (double)nx * ny * nz;
+    const bool big = cells > 67108864.0;
+    std::vector<int> txs = big ? std::vector<int>{4, 8} : std::vector<int>{2, 4, 8};
+    std::vector<int> lys = big ? std::vector<int>{16, 32, 64} : std::vector<int>{0, 1, 2, 4, 8, 16, 32, 64};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    TF_HIP(hipEventCreate(&e0));
+    TF_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    int btx = default_loop_tx(nx, ny, nz), bly = 0;
+    std::vector<std::tuple<int, int, int>> seen;  // (tx, resolved ly, line waves)
+    int rc = TF_OK;
+    for (int tx : txs) {
+        if (tx > 2 && tx > nx) continue;
+        for (int ly : lys) {
+            if (ly > ny) continue;
+            PlanGuard plan;
+            rc = tf_batch_create(&plan.b, &d, 1, plan_workspace(workspace), tf_batch_workspace_bytes(1), tx, ly,
+                                 stream);
+            if (rc != TF_OK) break;
+            const SimDev &S = plan.b->h_sims[0];
+            const auto key = std::make_tuple(tx, S.ly, S.line_w);
+            if (std::find(seen.begin(), seen.end(), key) != seen.end()) continue;
+            seen.push_back(key);
+            const int reps = big ? 2 : 5;
+            hipError_t e = launch_stencils(plan.b, st);
+            if (e == hipSuccess) e = hipEventRecord(e0, st);
+            for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_stencils(plan.b, st);
+            if (e == hipSuccess) e = hipEventRecord(e1, st);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float ms = 0.f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (e != hipSuccess) {
+                rc = fail(TF_ERR_HIP, "autotune launch: %s", hipGetErrorString(e));
+                break;
+            }
+            ms /= reps;
+            if (ms < best) {
+                best = ms;
+                btx = tx;
+                bly = S.ly;
+            }
+        }
+        if (rc != TF_OK) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != TF_OK) return rc;
+    {
+        std::lock_guard<std::mutex> lk(g_tiling_mu);
+        g_loop_tiling[{nx, ny, nz}] = {btx, bly};
+    }
+    *tile_x = btx;
+    *tile_y = bly;
+    if (ms_per_step) *ms_per_step = best;
+    return TF_OK;
+}

@@ -67,8 +67,20 @@ __device__ __forceinline__ T *sgpr_ptr(T *p) {
     return (T *)(((uint64_t)hi << 32) | lo);
 }
 
+#ifndef TF_LOAD_AUX_IN
+#define TF_LOAD_AUX_IN 0
+#endif
+#ifndef TF_LOAD_AUX_HALO
+#define TF_LOAD_AUX_HALO 0
+#endif
 __device__ __forceinline__ float buf_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+__device__ __forceinline__ float buf_load_in(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, TF_LOAD_AUX_IN));
+}
+__device__ __forceinline__ float buf_load_halo(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, TF_LOAD_AUX_HALO));
 }
 __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, float v) {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, (int)voff, (int)soff, TF_STORE_AUX);
@@ -119,7 +131,8 @@ struct HaloCtx {
             for (int f = 0; f < 5; ++f) w[f] = in_ptr[f * nc + ((size_t)soff + zb) / 4];
         } else {
 #pragma unroll
-            for (int f = 0; f < 5; ++f) w[f] = buf_load(rin[f], zb, soff);
+            for (int f = 0; f < 5; ++f)
+                w[f] = (j >= 2 && j < TX) ? buf_load_in(rin[f], zb, soff) : buf_load_halo(rin[f], zb, soff);
         }
     }
     // Z flux at the lower interface of this lane's cell (left state = lane - 1).

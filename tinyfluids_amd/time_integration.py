@@ -12,11 +12,34 @@ from __future__ import annotations
 
 import ctypes
 import math
+import os
 
 import torch
 
 from . import _lib
 from .fluid import _ptr, _scalar, _stream, check_state, workspace
+
+# Grid shapes whose loop tiling has been measured in this process. The first
+# run on a new shape times the candidate tilings on its own state (as JAX
+# compiles on the first call of a shape); TINYFLUIDS_AUTOTUNE=0 keeps the
+# untuned default. Results do not depend on the tiling.
+_TUNED = set()
+
+
+def autotune(primitive_state, grid_spacing, gamma, handle_boundaries=False, scratch=None):
+    """Measures the loop tilings of this state's shape (tf_autotune_loop) and
+    keeps the fastest for later runs. Returns (tile_x, tile_y, ms per step)."""
+    s = check_state(primitive_state)
+    _, nx, ny, nz = s.shape
+    scratch = torch.empty_like(s) if scratch is None else scratch
+    ws = workspace(nx, ny, nz, s.device)
+    tx, ty, ms = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_float(0.0)
+    _lib.check(_lib.load().tf_autotune_loop(
+        _ptr(s), _ptr(scratch), nx, ny, nz, _scalar(grid_spacing, "grid_spacing"), _scalar(gamma, "gamma"),
+        int(bool(handle_boundaries)), _ptr(ws), ctypes.byref(tx), ctypes.byref(ty), ctypes.byref(ms),
+        _stream(s.device)))
+    _TUNED.add((nx, ny, nz))
+    return tx.value, ty.value, ms.value
 
 
 def _run(primitive_state, grid_spacing, t_final, gamma, handle_boundaries, max_steps, C_CFL):
@@ -24,6 +47,8 @@ def _run(primitive_state, grid_spacing, t_final, gamma, handle_boundaries, max_s
     _, nx, ny, nz = s.shape
     state = s.clone()
     scratch = torch.empty_like(state)
+    if (nx, ny, nz) not in _TUNED and os.environ.get("TINYFLUIDS_AUTOTUNE", "1") != "0":
+        autotune(state, grid_spacing, gamma, handle_boundaries, scratch)
     ws = workspace(nx, ny, nz, state.device)
     n = ctypes.c_int64(0)
     t = ctypes.c_double(0.0)
