@@ -40,7 +40,7 @@ def parse():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=40)
-    ap.add_argument("--serve-rates", type=str, default="8,32", help="Poisson arrival rates (jobs/s) to sweep")
+    ap.add_argument("--serve-rates", type=str, default="8,32,128", help="Poisson arrival rates (jobs/s) to sweep")
     ap.add_argument("--no-harness", action="store_true")
     ap.add_argument("--harness-sizes", type=str, default="32,64,128,256,384,512,768")
     ap.add_argument("--no-shapes", action="store_true", help="skip the non-cubic 1536x512x512 roofline line")
@@ -204,12 +204,14 @@ def serve_leg(dev, njobs, rates):
     out = []
     for rate in rates:
         specs = ics.serving_stream(njobs, rate, seed=4)
-        srv = engine.Server(hbm_budget_bytes=64 << 30)
-        r = srv.serve(specs).summary()
+        r = engine.Server(hbm_budget_bytes=64 << 30).serve(specs).summary()
+        f = engine.Server(hbm_budget_bytes=64 << 30, policy="fcfs").serve(specs).summary()
         p = engine.serve_plain(specs).summary()
         out.append({"rate_jobs_per_s": rate, "jobs": njobs,
                     "sims_per_s": r["sims_per_s"], "p50_latency_s": r["p50_latency_s"],
                     "p99_latency_s": r["p99_latency_s"], "max_resident": r["max_resident"],
+                    "fcfs_batched": {"sims_per_s": f["sims_per_s"], "p50_latency_s": f["p50_latency_s"],
+                                     "p99_latency_s": f["p99_latency_s"]},
                     "plain_sims_per_s": p["sims_per_s"], "plain_p50_latency_s": p["p50_latency_s"],
                     "plain_p99_latency_s": p["p99_latency_s"],
                     "speedup_sims_per_s": r["sims_per_s"] / p["sims_per_s"]})

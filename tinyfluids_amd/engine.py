@@ -170,19 +170,28 @@ class ServeReport:
 class Server:
     """Continuous-batching server of simulation jobs on one GPU.
 
-    Every admitted job stays resident in HBM. Each tick advances a batch of
-    resident jobs by a number of steps in one multi-simulation launch
-    sequence. The batch always holds the oldest resident job (so every job
-    progresses), then the others shortest remaining work first (cells x steps
-    left) until it holds ``fill_cells`` cells, about the size at which one
-    launch saturates HBM: small jobs are batched together and finish without
-    waiting behind long ones, and the device stays full. A tick lasts about
-    ``tick_ms`` (steps estimated from the batch's cells), so that the host's
-    per-tick work stays small next to the device's; ``steps_per_tick``
-    fixes it instead."""
+    Every admitted job stays resident in HBM. Jobs are grouped by grid size
+    (< 2^20 cells, < 2^23, larger): all jobs of a batch advance one step per
+    batched step, so only jobs of similar per-step cost share a batch, and
+    each group runs its own batch on its own stream, concurrently with the
+    others. Each tick, every group advances by as many steps as fit in about
+    ``tick_ms`` at its own step cost (estimated from its cells), so small
+    jobs progress at their own pace while large ones fill the device.
+
+    Within a group, a batch always holds the oldest resident job (so every
+    job progresses), then the others shortest remaining work first (cells x
+    steps left; ``policy="fcfs"``: arrival order) until it holds
+    ``fill_cells`` cells, about the size at which one launch saturates HBM.
+    ``steps_per_tick`` > 0 fixes the steps per tick instead."""
+
+    GROUP_BOUNDS = (1 << 20, 1 << 23)
 
     def __init__(self, hbm_budget_bytes: int, max_resident: int = 1024, steps_per_tick: int = 0, device=None,
-                 keep_states: bool = False, tile_x: int = 0, fill_cells: int = 1 << 24, tick_ms: float = 4.0):
+                 keep_states: bool = False, tile_x: int = 0, fill_cells: int = 1 << 24, tick_ms: float = 4.0,
+                 policy: str = "shortest"):
+        if policy not in ("shortest", "fcfs"):
+            raise ValueError(f"policy must be 'shortest' or 'fcfs', got {policy!r}")
+        self.policy = policy
         self.budget = int(hbm_budget_bytes)
         self.max_resident = int(max_resident)
         self.steps_per_tick = int(steps_per_tick)
@@ -192,13 +201,19 @@ class Server:
         self.fill_cells = int(fill_cells)
         self.tick_ms = float(tick_ms)
 
-    def _select(self, active: Dict[int, Simulation], jobs: List[JobRecord]) -> List[int]:
-        """Resident jobs of the next tick (see the class description)."""
-        oldest = min(active, key=lambda i: (jobs[i].arrival, i))
+    @classmethod
+    def group_of(cls, spec: SimSpec) -> int:
+        return sum(spec.cells >= b for b in cls.GROUP_BOUNDS)
+
+    def _select(self, members: List[int], jobs: List[JobRecord]) -> List[int]:
+        """Jobs of a group's next tick (see the class description)."""
+        oldest = min(members, key=lambda i: (jobs[i].arrival, i))
         chosen, cells = [oldest], jobs[oldest].spec.cells
-        rest = sorted((i for i in active if i != oldest),
-                      key=lambda i: (jobs[i].spec.cells * (jobs[i].spec.steps - jobs[i].steps_done), i))
-        for i in rest:
+        if self.policy == "fcfs":
+            key = lambda i: (jobs[i].arrival, i)  # noqa: E731
+        else:
+            key = lambda i: (jobs[i].spec.cells * (jobs[i].spec.steps - jobs[i].steps_done), i)  # noqa: E731
+        for i in sorted((i for i in members if i != oldest), key=key):
             if cells >= self.fill_cells:
                 break
             chosen.append(i)
@@ -223,25 +238,24 @@ class Server:
             if state_bytes(j.spec) > self.budget:
                 raise ValueError(f"job {j.index} ({j.spec.shape}) does not fit the HBM budget")
         tune_plain(specs, self.device)
+        ngroups = len(self.GROUP_BOUNDS) + 1
+        streams = [torch.cuda.Stream(self.device) for _ in range(ngroups)]  # default priority
         pending = list(jobs)  # in arrival order
         queue: List[JobRecord] = []
         active: Dict[int, Simulation] = {}
-        used = 0
-        peak = 0
-        ticks = 0
-        max_res = 0
-        batch: Optional[Batch] = None
-        ids: List[int] = []
-        dirty = False
+        batches: List[Optional[Batch]] = [None] * ngroups
+        ids: List[List[int]] = [[] for _ in range(ngroups)]
+        used = peak = ticks = max_res = done = 0
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        done = 0
         while done < len(jobs):
-            # 1. enqueue the next tick of the resident jobs (asynchronous)
-            ticked = batch is not None
-            if ticked:
-                batch.step(self._tick_steps(ids, jobs))
-                ticks += 1
+            # 1. enqueue the next tick of every group (asynchronous, one stream each)
+            ticked = [b is not None for b in batches]
+            for g, b in enumerate(batches):
+                if b is not None:
+                    with torch.cuda.stream(streams[g]):
+                        b.step(self._tick_steps(ids[g], jobs))
+            ticks += any(ticked)
             # 2. admission in arrival order while memory and slots allow; the
             #    host prepares new jobs while the device runs the tick
             now = time.perf_counter() - t0
@@ -250,46 +264,54 @@ class Server:
             while queue and len(active) < self.max_resident and used + state_bytes(queue[0].spec) <= self.budget:
                 j = queue.pop(0)
                 s = j.spec
-                sim = Simulation(s.initial_state(), s.dx, s.gamma, max_steps=s.steps, device=self.device)
-                sim.reset()
+                with torch.cuda.stream(streams[self.group_of(s)]):
+                    sim = Simulation(s.initial_state(), s.dx, s.gamma, max_steps=s.steps, device=self.device)
+                    sim.reset()
                 active[j.index] = sim
                 used += state_bytes(s)
                 j.admitted = time.perf_counter() - t0
-                dirty = True
             peak = max(peak, used)
             max_res = max(max_res, len(active))
-            # 3. the tick's outcome: finished jobs leave and free their memory
-            if ticked:
-                statuses = batch.status()  # waits for the tick
-                now = time.perf_counter() - t0
-                for i, st in zip(ids, statuses):
-                    jobs[i].steps_done = int(st.n_iter)
-                    if not st.active:
-                        j = jobs[i]
-                        sim = active.pop(i)
-                        j.finished = now
-                        j.steps_done = int(st.n_iter)
-                        j.nonfinite = bool(st.nonfinite)
-                        if self.keep_states:
-                            j.state = sim.state(st).clone()
-                        used -= state_bytes(j.spec)
-                        done += 1
-                        dirty = True
-            # 4. a new plan when the chosen jobs change
-            chosen = self._select(active, jobs) if active else []
-            if dirty or chosen != ids:
-                if batch is not None:
-                    batch.close()
-                    batch = None
-                ids = chosen
-                if ids:
-                    batch = Batch([active[i] for i in ids], tile_x=self.tile_x)
-                dirty = False
-            if batch is None and pending and not queue:
+            # 3. each group's tick outcome: finished jobs leave and free their memory
+            for g, b in enumerate(batches):
+                if not ticked[g]:
+                    continue
+                with torch.cuda.stream(streams[g]):
+                    statuses = b.status()  # waits for this group's tick
+                    now = time.perf_counter() - t0
+                    for i, st in zip(ids[g], statuses):
+                        jobs[i].steps_done = int(st.n_iter)
+                        if not st.active:
+                            j = jobs[i]
+                            sim = active.pop(i)
+                            j.finished = now
+                            j.nonfinite = bool(st.nonfinite)
+                            if self.keep_states:
+                                j.state = sim.state(st).clone()
+                            used -= state_bytes(j.spec)
+                            done += 1
+            # 4. a group's plan is rebuilt when its chosen jobs change
+            members = [[] for _ in range(ngroups)]
+            for i in active:
+                members[self.group_of(jobs[i].spec)].append(i)
+            for g in range(ngroups):
+                chosen = self._select(members[g], jobs) if members[g] else []
+                if chosen != ids[g] or (batches[g] is None and chosen):
+                    with torch.cuda.stream(streams[g]):
+                        if batches[g] is not None:
+                            batches[g].close()
+                            batches[g] = None
+                        ids[g] = chosen
+                        if chosen:
+                            batches[g] = Batch([active[i] for i in chosen], tile_x=self.tile_x)
+            if all(b is None for b in batches) and pending and not queue:
                 # idle: wait for the next arrival
                 time.sleep(max(0.0, min(0.01, pending[0].arrival - (time.perf_counter() - t0))))
-        if batch is not None:
-            batch.close()
+        for g, b in enumerate(batches):
+            if b is not None:
+                with torch.cuda.stream(streams[g]):
+                    b.close()
+        torch.cuda.synchronize()
         makespan = time.perf_counter() - t0
         return ServeReport(jobs, makespan, ticks, max_res, peak)
 
