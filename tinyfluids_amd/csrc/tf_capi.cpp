@@ -180,19 +180,25 @@ SimDev make_simdev(int nx, int ny, int nz, int open_bc, double gamma, int tx, in
 
 // The tiles of S that touch the last (x, y) row (the device-side predicate
 // touches_last_row, evaluated for every tile); appended for simulation `sim`.
-void far_tiles(const SimDev &S, int tx, int sim, std::vector<FarTile> &out) {
+// With `split`, each tile's rows are cut into up to kFarSplit ranges (of at
+// least 4 rows): the far kernel runs after the main one, so its waves should
+// be many and short.
+void far_tiles(const SimDev &S, int tx, int sim, std::vector<FarTile> &out, bool split) {
     if (!S.far) return;
     for (int yc = 0; yc < S.nyc; ++yc)
         for (int xt = 0; xt < S.ntx; ++xt) {
             const bool xr = xt == 0 || xt * tx + tx >= S.nx - 1;
             const bool yr = yc == 0 || yc * S.ly + S.ly >= S.ny - 1;
-            if (xr && yr) out.push_back(FarTile{sim, xt, yc, 0});
+            if (!(xr && yr)) continue;
+            const int y0 = yc * S.ly, rows = std::min(S.ly, S.ny - y0);
+            const int sub = split ? std::max(4, (rows + kFarSplit - 1) / kFarSplit) : rows;
+            for (int r = 0; r < rows; r += sub) out.push_back(FarTile{sim, xt, y0 + r, std::min(sub, rows - r)});
         }
 }
 
 FarList far_list(const SimDev &S, int tx) {
     std::vector<FarTile> v;
-    far_tiles(S, tx, 0, v);
+    far_tiles(S, tx, 0, v, false);
     FarList F;
     std::memset(&F, 0, sizeof F);
     F.n = (int)std::min<size_t>(v.size(), kMaxFarPerSim);
@@ -426,7 +432,8 @@ static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 size_t tf_batch_workspace_bytes(int nsims) {
     if (nsims < 0) nsims = 0;
     return align256((size_t)nsims * sizeof(SimDev)) + align256((size_t)(nsims + 5) * sizeof(int)) +
-           align256((size_t)nsims * sizeof(Ctrl)) + align256((size_t)nsims * kMaxFarPerSim * sizeof(FarTile));
+           align256((size_t)nsims * sizeof(Ctrl)) +
+           align256((size_t)nsims * kMaxFarPerSim * kFarSplit * sizeof(FarTile));
 }
 
 namespace {
@@ -608,7 +615,7 @@ int tf_batch_create(tf_batch **out, const tf_sim_desc *sims, int nsims, void *wo
     std::vector<FarTile> far;
     for (int k = 0; k < nsims; ++k) {
         const size_t before = far.size();
-        far_tiles(b->h_sims[k], tx, k, far);
+        far_tiles(b->h_sims[k], tx, k, far, true);
         if (far.size() > before) b->far_max_ntz = std::max(b->far_max_ntz, b->h_sims[k].ntz);
     }
     b->nfar = (int)far.size();

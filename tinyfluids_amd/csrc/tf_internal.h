@@ -45,11 +45,13 @@ struct SimDev {
     int groups;             // line tiling: x-column groups per wave (nz <= 32 packs several)
 };
 
-// A tile handled by the far-row kernel: simulation index, x-tile, y-chunk.
+// A tile handled by the far-row kernel: simulation index, x-tile, and the
+// rows [y0, y0 + rows) it writes.
 struct FarTile {
-    int sim, xt, yc, pad;
+    int sim, xt, y0, rows;
 };
 constexpr int kMaxFarPerSim = 9;  // at most 3 x-tiles x 3 y-chunks touch the last row
+constexpr int kFarSplit = 8;      // batch plans split each far tile into up to 8 shorter row ranges
 
 struct FarList {
     int n;

@@ -309,7 +309,7 @@ __device__ __forceinline__ uint32_t stream_tile(const Ctx &T, const Gas &g, uint
 // the wave-local max of the signal-speed bits of the cells it wrote.
 template <int TX, bool FAR>
 __device__ __forceinline__ uint32_t fused_tile(const float *in_, float *out_, int nx, int ny, int nz, int open_bc,
-                                               const Gas &g_in, int xt, int zt, int yc, int ly) {
+                                               const Gas &g_in, int xt, int zt, int y0, int rows) {
     const Gas g{sgpr_f(g_in.gamma), sgpr_f(g_in.gm1), sgpr_f(g_in.inv_gm1), sgpr_f(g_in.k)};
     // Everything below is wave-uniform; readfirstlane keeps it in SGPRs even
     // where it was read from memory with vector loads.
@@ -319,8 +319,8 @@ __device__ __forceinline__ uint32_t fused_tile(const float *in_, float *out_, in
     open_bc = __builtin_amdgcn_readfirstlane(open_bc);
     xt = __builtin_amdgcn_readfirstlane(xt);
     zt = __builtin_amdgcn_readfirstlane(zt);
-    yc = __builtin_amdgcn_readfirstlane(yc);
-    ly = __builtin_amdgcn_readfirstlane(ly);
+    y0 = __builtin_amdgcn_readfirstlane(y0);
+    rows = __builtin_amdgcn_readfirstlane(rows);
     HaloCtx<TX, FAR> T;
     const size_t nc = (size_t)nx * ny * nz;
     {
@@ -338,11 +338,11 @@ __device__ __forceinline__ uint32_t fused_tile(const float *in_, float *out_, in
     const int lane = lane_id();
     T.x0 = xt * TX;
     T.z0 = zt * kZT;
-    T.y0 = yc * ly;
+    T.y0 = y0;
     T.ny = ny;
     T.open_bc = open_bc;
     const int nzo = min(kZT, nz - T.z0);
-    T.nrows = min(ly, ny - T.y0) + 2;
+    T.nrows = rows + 2;
     T.nxo = min(TX, nx - T.x0);
     T.row_b = 4u * (uint32_t)nz;
     T.plane_b = T.row_b * (uint32_t)ny;
@@ -478,7 +478,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_pe
     int xt, zt, yc;
     decode_tile(w, S.ntz, S.ntx, xt, zt, yc);
     if (S.far && touches_last_row<TX>(S, xt, yc)) return;  // k_step_direct_far does it
-    uint32_t v = fused_tile<TX, false>(in, out, S.nx, S.ny, S.nz, S.open_bc, g, xt, zt, yc, S.ly);
+    const int y0 = yc * S.ly;
+    uint32_t v = fused_tile<TX, false>(in, out, S.nx, S.ny, S.nz, S.open_bc, g, xt, zt, y0, min(S.ly, S.ny - y0));
     if (bits_out) {
         v = wave_umax(v);
         if (lane_id() == 0) atomicMax(bits_out, v);
@@ -494,7 +495,7 @@ __global__ __launch_bounds__(kBlock) void k_step_direct_far(SimDev S, FarList F,
     if (w >= F.n * S.ntz) return;
     const FarTile &t = F.t[w / S.ntz];
     Gas g{S.gamma, S.gm1, S.inv_gm1, (float)(*dt / dx)};
-    uint32_t v = fused_tile<TX, true>(in, out, S.nx, S.ny, S.nz, S.open_bc, g, t.xt, w % S.ntz, t.yc, S.ly);
+    uint32_t v = fused_tile<TX, true>(in, out, S.nx, S.ny, S.nz, S.open_bc, g, t.xt, w % S.ntz, t.y0, t.rows);
     if (bits_out) {
         v = wave_umax(v);
         if (lane_id() == 0) atomicMax(bits_out, v);
@@ -521,12 +522,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_pe
     int xt, zt, yc;
     decode_tile(w - prefix[lo], S.ntz, S.ntx, xt, zt, yc);
     if (S.far && touches_last_row<TX>(S, xt, yc)) return;  // k_step_sims_far does it
-    uint32_t v = fused_tile<TX, false>(S.buf[par], S.buf[par ^ 1], S.nx, S.ny, S.nz, S.open_bc, g, xt, zt, yc, S.ly);
+    const int y0 = yc * S.ly;
+    uint32_t v = fused_tile<TX, false>(S.buf[par], S.buf[par ^ 1], S.nx, S.ny, S.nz, S.open_bc, g, xt, zt, y0,
+                                       min(S.ly, S.ny - y0));
     v = wave_umax(v);
     if (lane_id() == 0) atomicMax(&C->max_bits[par ^ 1], v);
 }
 
-// The far tiles of k_step_sims: `far` lists (simulation, x-tile, y-chunk);
+// The far tiles of k_step_sims: `far` lists (simulation, x-tile, row range);
 // one wave per (listed tile, z-tile), max_ntz waves per entry.
 template <int TX>
 __global__ __launch_bounds__(kBlock) void k_step_sims_far(const SimDev *__restrict__ sims,
@@ -541,8 +544,8 @@ __global__ __launch_bounds__(kBlock) void k_step_sims_far(const SimDev *__restri
     if (!C->active) return;
     const int par = (int)(C->n_iter & 1);
     Gas g{S.gamma, S.gm1, S.inv_gm1, C->k};
-    uint32_t v = fused_tile<TX, true>(S.buf[par], S.buf[par ^ 1], S.nx, S.ny, S.nz, S.open_bc, g, t.xt, zt, t.yc,
-                                      S.ly);
+    uint32_t v = fused_tile<TX, true>(S.buf[par], S.buf[par ^ 1], S.nx, S.ny, S.nz, S.open_bc, g, t.xt, zt, t.y0,
+                                      t.rows);
     v = wave_umax(v);
     if (lane_id() == 0) atomicMax(&C->max_bits[par ^ 1], v);
 }
