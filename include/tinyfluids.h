@@ -155,10 +155,17 @@ size_t tf_batch_workspace_bytes(int nsims);
 
 /* Builds a plan; its device tables live in the caller's workspace.
  * tile_x (x-cells per wave: 2, 4, 6 or 8) and tile_y (rows per y-chunk) choose
- * the stencil tiling; 0 picks the library default. */
+ * the stencil tiling; 0 picks the library default. Grids with nz <= 1024 run
+ * whole z-lines per workgroup (1..16 waves) where that uses the lanes at
+ * least as well as 62-cell z-tiles, with tile width min(tile_x, 4), or 2
+ * above 8 waves. */
 int tf_batch_create(tf_batch **out, const tf_sim_desc *sims, int nsims, void *workspace, size_t workspace_bytes,
                     int tile_x, int tile_y, void *stream);
 int tf_batch_destroy(tf_batch *b);
+
+/* The tiling a plan runs simulation `sim` (caller's order) with: tile width,
+ * y-chunk rows, and waves per z-line (0: 62-cell z-tiles of one wave). */
+int tf_batch_sim_tiling(const tf_batch *b, int sim, int *tile_x, int *tile_y, int *line_waves);
 
 /* Enqueues `steps` batched steps: each is one fused multi-simulation stencil
  * launch over every simulation's tiles plus one per-simulation dt update.

@@ -113,14 +113,28 @@ def test_server_respects_hbm_budget():
 
 LINE_SHAPES = [((5, 6, 8), False), ((7, 5, 16), True), ((6, 4, 24), False), ((4, 9, 32), False),
                ((3, 5, 48), True), ((6, 3, 64), False), ((3, 4, 100), False), ((4, 3, 128), True),
-               ((3, 3, 192), False), ((2, 3, 256), False), ((9, 2, 20), False)]
+               ((3, 3, 192), False), ((2, 3, 256), False), ((9, 2, 20), False),
+               # 5..16 waves per z-line
+               ((3, 4, 300), False), ((4, 3, 512), True), ((2, 5, 520), False), ((2, 4, 700), True),
+               ((3, 3, 1024), False), ((2, 3, 1000), True)]
+
+
+def halo_path_steps(x, dx, gamma, n, hb):
+    """n steps through the single-step entry points: the CFL dt, then the
+    fused step, which always runs 62-cell z-tiles."""
+    import tinyfluids_amd as tf
+    for _ in range(n):
+        dt = tf._cfl_time_step(x, dx, gamma)
+        x = tf._evolve_state(x, dx, dt, gamma, hb)
+    return x
 
 
 @pytest.mark.parametrize("line_mode", [1, 0])
 @pytest.mark.parametrize("tile_x", [2, 4, 8])
 def test_line_tiling_bitwise(line_mode, tile_x):
-    """Batched plans with whole-z-line workgroups (W = 1..4 waves, several
-    x-groups per wave for nz <= 32) give the plain loop's results bit for bit."""
+    """Batched plans with whole-z-line workgroups (W = 1..16 waves, several
+    x-groups per wave for nz <= 32) give the plain loop's results bit for
+    bit, and both equal the 62-cell z-tiling of the single-step path."""
     import tinyfluids_amd._lib as L
     from tinyfluids_amd.batch import Batch, Simulation
     from tinyfluids_amd.time_integration import run_steps
@@ -148,6 +162,8 @@ def test_line_tiling_bitwise(line_mode, tile_x):
         ref, _, _ = run_steps(x, 0.05, 1.4, 4, handle_boundaries=hb)
         assert st[i].n_iter == 4
         assert torch.equal(sim.state(st[i]), ref), (LINE_SHAPES[i], line_mode, tile_x)
+        if line_mode == 1 and tile_x == 4:
+            assert torch.equal(halo_path_steps(x, 0.05, 1.4, 4, hb), ref), LINE_SHAPES[i]
 
 
 def test_tune_batch_picks_a_measured_candidate():

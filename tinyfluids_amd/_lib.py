@@ -70,6 +70,7 @@ SIGNATURES = {
     "tf_batch_workspace_bytes": (SZ, [I]),
     "tf_batch_create": (I, [ctypes.POINTER(P), ctypes.POINTER(SimDesc), I, P, SZ, I, I, P]),
     "tf_batch_destroy": (I, [P]),
+    "tf_batch_sim_tiling": (I, [P, I, ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I)]),
     "tf_batch_step": (I, [P, I, P]),
     "tf_batch_step_graph": (I, [P, I, P]),
     "tf_batch_status": (I, [P, ctypes.POINTER(SimStatus), P]),

@@ -102,6 +102,13 @@ class Batch:
         except Exception:
             pass
 
+    def tiling(self, index: int = 0):
+        """(tile_x, tile_y, waves per z-line) the plan runs simulation `index` with."""
+        tx, ty, w = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+        _lib.check(_lib.load().tf_batch_sim_tiling(self._h, int(index), ctypes.byref(tx), ctypes.byref(ty),
+                                                   ctypes.byref(w)))
+        return tx.value, ty.value, w.value
+
     def step(self, steps: int = 1):
         _lib.check(_lib.load().tf_batch_step(self._h, int(steps), _stream(self.device)))
 

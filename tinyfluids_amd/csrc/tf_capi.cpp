@@ -184,7 +184,7 @@ SimDev make_simdev(int nx, int ny, int nz, int open_bc, double gamma, int tx, in
 // least 4 rows): the far kernel runs after the main one, so its waves should
 // be many and short.
 void far_tiles(const SimDev &S, int tx, int sim, std::vector<FarTile> &out, bool split) {
-    if (!S.far) return;
+    if (!S.far || S.line_w > 0) return;  // line-tiled grids reach the last cell themselves
     for (int yc = 0; yc < S.nyc; ++yc)
         for (int xt = 0; xt < S.ntx; ++xt) {
             const bool xr = xt == 0 || xt * tx + tx >= S.nx - 1;
@@ -401,8 +401,12 @@ int tf_autotune_evolve(const float *in, float *out, int nx, int ny, int nz, doub
 // ------------------------------------------------------------------ batched engine
 // A class of simulations launched together: halo tiling (w == 0, prefix over
 // waves) or line tiling with w waves per workgroup (prefix over workgroups).
+constexpr int kMaxLineW = 16;          // whole z-lines of up to 1024 cells per workgroup
+constexpr int kClasses = kMaxLineW + 1;  // halo tiling + line tiling with W = 1..16
+
 struct SimClass {
     int w = 0;
+    int tx = 0;  // tile width of the class's kernel
     int begin = 0, count = 0, total = 0;
     int *d_prefix = nullptr;
 };
@@ -414,7 +418,7 @@ struct tf_batch {
     SimDev *d_sims = nullptr;
     int *d_prefix = nullptr;
     Ctrl *d_status = nullptr;
-    SimClass cls[5];
+    SimClass cls[kClasses];
     std::vector<SimDev> h_sims;  // device table order
     std::vector<int> h_prefix;
     std::vector<int> order;      // h_sims[k] is the caller's simulation order[k]
@@ -431,7 +435,7 @@ static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 size_t tf_batch_workspace_bytes(int nsims) {
     if (nsims < 0) nsims = 0;
-    return align256((size_t)nsims * sizeof(SimDev)) + align256((size_t)(nsims + 5) * sizeof(int)) +
+    return align256((size_t)nsims * sizeof(SimDev)) + align256((size_t)(nsims + kClasses) * sizeof(int)) +
            align256((size_t)nsims * sizeof(Ctrl)) +
            align256((size_t)nsims * kMaxFarPerSim * kFarSplit * sizeof(FarTile));
 }
@@ -446,6 +450,9 @@ void line_shape(int nz, int &w, int &groups) {
     groups = w == 1 ? std::max(1, 64 / nz) : 1;
 }
 double halo_lane_use(int nz) { return (double)nz / (64.0 * ((nz + kZT - 1) / kZT)); }
+// Tile width of a line class: at most 4 (registers for two waves per SIMD),
+// and 2 above 8 waves per workgroup (four waves per SIMD).
+int line_tx(int w, int tx) { return w > 8 ? 2 : std::min(tx, 4); }
 double line_lane_use(int nz) {
     int w, g;
     line_shape(nz, w, g);
@@ -527,12 +534,16 @@ int tf_batch_create(tf_batch **out, const tf_sim_desc *sims, int nsims, void *wo
             S.buf[0] = d.buf0;
             S.buf[1] = d.buf1;
             S.ctrl = reinterpret_cast<Ctrl *>(d.ctrl);
-            if (line_on && !S.far && d.nz <= 256 && line_lane_use(d.nz) > halo_lane_use(d.nz) + 1e-9) {
+            // Whole z-lines whenever they fit a workgroup and use the lanes at
+            // least as well: every row segment then starts at its line's
+            // start instead of one cell before a 62-cell tile.
+            if (line_on && d.nz <= 64 * kMaxLineW && line_lane_use(d.nz) >= halo_lane_use(d.nz) - 1e-9) {
                 int w, groups;
                 line_shape(d.nz, w, groups);
+                const int txl = line_tx(w, tx);
                 S.line_w = w;
                 S.groups = groups;
-                S.ntx = (d.nx + groups * tx - 1) / (groups * tx);
+                S.ntx = (d.nx + groups * txl - 1) / (groups * txl);
                 S.ntz = 1;
                 S.waves = S.ntx * S.nyc * w;
             }
@@ -586,10 +597,11 @@ int tf_batch_create(tf_batch **out, const tf_sim_desc *sims, int nsims, void *wo
     // classes: contiguous table ranges, each with its own prefix (waves or workgroups)
     b->h_prefix.clear();
     b->total_waves = 0;
-    std::vector<int> cls_prefix_at(5, 0);
-    for (int c = 0, k = 0; c < 5; ++c) {
+    std::vector<int> cls_prefix_at(kClasses, 0);
+    for (int c = 0, k = 0; c < kClasses; ++c) {
         SimClass &C = b->cls[c];
         C.w = c;
+        C.tx = c == 0 ? tx : line_tx(c, tx);
         C.begin = k;
         cls_prefix_at[c] = (int)b->h_prefix.size();
         b->h_prefix.push_back(0);
@@ -607,8 +619,8 @@ int tf_batch_create(tf_batch **out, const tf_sim_desc *sims, int nsims, void *wo
     b->d_sims = reinterpret_cast<SimDev *>(p);
     p += align256((size_t)nsims * sizeof(SimDev));
     b->d_prefix = reinterpret_cast<int *>(p);
-    for (int c = 0; c < 5; ++c) b->cls[c].d_prefix = b->d_prefix + cls_prefix_at[c];
-    p += align256((size_t)(nsims + 5) * sizeof(int));
+    for (int c = 0; c < kClasses; ++c) b->cls[c].d_prefix = b->d_prefix + cls_prefix_at[c];
+    p += align256((size_t)(nsims + kClasses) * sizeof(int));
     b->d_status = reinterpret_cast<Ctrl *>(p);
     p += align256((size_t)nsims * sizeof(Ctrl));
     b->d_far = reinterpret_cast<FarTile *>(p);
@@ -635,6 +647,20 @@ int tf_batch_create(tf_batch **out, const tf_sim_desc *sims, int nsims, void *wo
     return TF_OK;
 }
 
+int tf_batch_sim_tiling(const tf_batch *b, int sim, int *tile_x, int *tile_y, int *line_waves) {
+    TF_REQUIRE(b && tile_x && tile_y && line_waves, "null pointer");
+    TF_REQUIRE(sim >= 0 && sim < b->nsims, "simulation index %d out of range", sim);
+    for (int k = 0; k < b->nsims; ++k) {
+        if (b->order[k] != sim) continue;
+        const SimDev &S = b->h_sims[k];
+        *line_waves = S.line_w;
+        *tile_x = b->cls[S.line_w].tx;
+        *tile_y = S.ly;
+        return TF_OK;
+    }
+    return fail(TF_ERR_ARG, "simulation %d not in the plan", sim);
+}
+
 int tf_batch_destroy(tf_batch *b) {
     if (!b) return TF_OK;
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
@@ -650,9 +676,9 @@ static hipError_t launch_stencils(tf_batch *b, hipStream_t st) {
     const SimClass &H = b->cls[0];
     hipError_t e = hipSuccess;
     if (H.count > 0) e = launch_step_sims(b->tx, b->d_sims + H.begin, H.d_prefix, H.count, H.total, st);
-    for (int c = 1; c < 5 && e == hipSuccess; ++c) {
+    for (int c = 1; c < kClasses && e == hipSuccess; ++c) {
         const SimClass &L = b->cls[c];
-        if (L.count > 0) e = launch_line_sims(b->tx, c, b->d_sims + L.begin, L.d_prefix, L.count, L.total, st);
+        if (L.count > 0) e = launch_line_sims(L.tx, c, b->d_sims + L.begin, L.d_prefix, L.count, L.total, st);
     }
     if (e == hipSuccess) e = launch_step_sims_far(b->tx, b->d_sims, b->d_far, b->nfar, b->far_max_ntz, st);
     return e;
@@ -861,7 +887,7 @@ int tf_autotune_loop(float *state, float *scratch, int nx, int ny, int nz, doubl
                                  stream);
             if (rc != TF_OK) break;
             const SimDev &S = plan.b->h_sims[0];
-            const auto key = std::make_tuple(tx, S.ly, S.line_w);
+            const auto key = std::make_tuple(S.line_w > 0 ? line_tx(S.line_w, tx) : tx, S.ly, S.line_w);
             if (std::find(seen.begin(), seen.end(), key) != seen.end()) continue;
             seen.push_back(key);
             const int reps = big ? 2 : 5;

@@ -88,6 +88,19 @@ __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t vof
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t field_rsrc(const float *field) {
     return __builtin_amdgcn_make_buffer_rsrc((void *)field, (short)0, -1, 0x00020000);
 }
+// Structured (indexed, stride 4) buffer access. Its range check is on the
+// element index only (index < num_records), so every dword of a field of up
+// to 2^30 cells is reachable, the last one of an exactly 4 GiB field
+// included; the scalar byte offset is added unchecked. The LLVM intrinsics
+// have no clang builtin, so they are bound by name.
+extern "C" __device__ float tf_struct_load_f32(__amdgpu_buffer_rsrc_t rsrc, int vindex, int voffset, int soffset,
+                                               int aux) __asm("llvm.amdgcn.struct.ptr.buffer.load.f32");
+extern "C" __device__ void tf_struct_store_f32(float v, __amdgpu_buffer_rsrc_t rsrc, int vindex, int voffset,
+                                               int soffset, int aux) __asm("llvm.amdgcn.struct.ptr.buffer.store.f32");
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t field_rsrc_indexed(const float *field) {
+    return __builtin_amdgcn_make_buffer_rsrc((void *)field, (short)4, -1, 0x00020000);
+}
+
 // An empty asm that takes v in a VGPR: v is computed before this point on
 // the lanes active here, and its computation cannot be sunk into a later
 // lane-masked block. Emits no instruction.
@@ -147,6 +160,12 @@ struct HaloCtx {
 #pragma unroll
         for (int i = 0; i < 5; ++i) out[i] = fmaf(k, F[i] - from_right(F[i]), U[i]);
     }
+    // Z sweep of the lane's cell: out = U + k (F[z-1/2] - F[z+1/2]).
+    __device__ __forceinline__ void zsweep(const Q &q, float k, float (&out)[5], int s) const {
+        float F[5];
+        zflux(q, F, s);
+        zupdate(q.U, F, k, out, s);
+    }
     __device__ __forceinline__ bool out_ok(int s) const { return zout && s < nxo; }
     __device__ __forceinline__ void store(int s, int y, const float (&w)[5]) const {
         const uint32_t so = (uint32_t)(x0 + s) * plane_b + (uint32_t)y * row_b + 4u * (uint32_t)z0;
@@ -161,62 +180,69 @@ struct HaloCtx {
 };
 
 // ---------------------------------------------------------------------------
-// LineCtx: a workgroup of W waves holds whole periodic z-lines (nz <= 64 W),
-// so there are no z-halo columns. With W = 1 and nz <= 32 the wave packs
-// G = 64/nz groups of lanes, each on its own TX x-columns. Lane gl = (group
-// g, z); z-neighbours are exchanged through LDS with periodic indices, two
-// barriers per cell row (all waves of the group run the same rows). Loads:
-// per-lane voffset (x-row and z, fixed per tile) + scalar y-row offset.
+// LineCtx: a workgroup of W waves (W <= 16) holds whole periodic z-lines
+// (nz <= 64 W), so there are no z-halo columns and every row segment a wave
+// loads or stores starts at its z-line's start. With W = 1 and nz <= 32 the
+// wave packs G = 64/nz groups of lanes, each on its own TX x-columns. Lane
+// gl = (group g, z); z-neighbours are exchanged through LDS with periodic
+// indices, one barrier per cell of a row (all waves of the group run the
+// same rows). Loads and stores are indexed buffer accesses: a per-lane cell index
+// (x-row and z, fixed per tile) plus a scalar byte offset for the y-row.
 // ---------------------------------------------------------------------------
-template <int TX, int W>
+template <int TX>
 struct LineCtx {
     __amdgpu_buffer_rsrc_t rin[5], rout[5];
-    uint32_t xz[TX + 2];  // per-lane byte offsets of the loaded columns (x-row, z; wrapped/clamped)
-    uint32_t xs_b;        // per-lane byte offset of the lane's own cell in slot 0 (unclamped x, z)
+    uint32_t xi[TX + 2];  // per-lane cell indices of the loaded columns in row 0 (x-row, z; wrapped/clamped)
+    uint32_t xs_i;        // per-lane cell index of the lane's own cell in row 0 (unclamped x, z)
     int ny, y0, nrows, open_bc;
     uint32_t row_b, plane_b;
     int lidx, ridx, gl;   // LDS slots: left / right neighbour, own
     int xs0;              // first x-column of this lane's group
     int nx;
+    int lanes;            // 64 W: LDS stride of one quantity
     bool valid;
-    float *lq;            // LDS: 12 x 64W floats per slot parity
-    float *lf;            // LDS: 5 x 64W floats per slot parity
+    float *lq;            // LDS: 13 x 64W floats per slot parity
+    static constexpr int kLq = 13;
 
     __device__ __forceinline__ void load(int j, uint32_t yoff, float (&w)[5]) const {
 #pragma unroll
-        for (int f = 0; f < 5; ++f) w[f] = buf_load(rin[f], xz[j], yoff);
+        for (int f = 0; f < 5; ++f) w[f] = tf_struct_load_f32(rin[f], (int)xi[j], 0, (int)yoff, 0);
     }
-    // Same arithmetic as HaloCtx::zflux / zupdate, neighbours through LDS.
-    __device__ __forceinline__ void zflux(const Q &q, float (&F)[5], int s) const {
-        float *b = lq + (s & 1) * 12 * 64 * W;
+    // The Z sweep with one LDS exchange per column: every lane publishes its
+    // sweep quantities, then computes the fluxes at both of its interfaces
+    // (the same HLL expression, operands in the same order, as the
+    // neighbour that shares the interface), so no flux exchange and no second
+    // barrier is needed. Each lane's 12 quantities are consecutive with one
+    // float of padding: every LDS access is a lane base plus an immediate
+    // offset, and the odd stride keeps the 64 lanes on distinct banks. Two
+    // slots alternate by column parity, so one barrier per column suffices.
+    __device__ __forceinline__ void zsweep(const Q &q, float k, float (&out)[5], int s) const {
+        float *b = lq + (s & 1) * kLq * lanes;
+        float *own = b + gl * kLq;
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
-            b[i * 64 * W + gl] = q.U[i];
-            b[(5 + i) * 64 * W + gl] = q.F[i];
+            own[i] = q.U[i];
+            own[5 + i] = q.F[i];
         }
-        b[10 * 64 * W + gl] = q.up;
-        b[11 * 64 * W + gl] = q.um;
+        own[10] = q.up;
+        own[11] = q.um;
         __syncthreads();
-        const HllCoef c =
-            hll_coef(fmax3(b[10 * 64 * W + lidx], q.up, 0.0f), fmin3(b[11 * 64 * W + lidx], q.um, 0.0f));
+        const float *L = b + lidx * kLq;
+        const float *R = b + ridx * kLq;
+        const HllCoef cl = hll_coef(fmax3(L[10], q.up, 0.0f), fmin3(L[11], q.um, 0.0f));
+        const HllCoef cr = hll_coef(fmax3(q.up, R[10], 0.0f), fmin3(q.um, R[11], 0.0f));
 #pragma unroll
-        for (int i = 0; i < 5; ++i)
-            F[i] = hll_comp(c, q.U[i] - b[i * 64 * W + lidx], c.a * b[(5 + i) * 64 * W + lidx], q.F[i]);
-    }
-    __device__ __forceinline__ void zupdate(const float (&U)[5], const float (&F)[5], float k, float (&out)[5],
-                                            int s) const {
-        float *b = lf + (s & 1) * 5 * 64 * W;
-#pragma unroll
-        for (int i = 0; i < 5; ++i) b[i * 64 * W + gl] = F[i];
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < 5; ++i) out[i] = fmaf(k, F[i] - b[i * 64 * W + ridx], U[i]);
+        for (int i = 0; i < 5; ++i) {
+            const float fl = hll_comp(cl, q.U[i] - L[i], cl.a * L[5 + i], q.F[i]);
+            const float fr = hll_comp(cr, R[i] - q.U[i], cr.a * q.F[i], R[5 + i]);
+            out[i] = fmaf(k, fl - fr, q.U[i]);
+        }
     }
     __device__ __forceinline__ bool out_ok(int s) const { return valid && xs0 + s < nx; }
     __device__ __forceinline__ void store(int s, int y, const float (&w)[5]) const {
         const uint32_t so = (uint32_t)y * row_b + (uint32_t)s * plane_b;
 #pragma unroll
-        for (int f = 0; f < 5; ++f) buf_store(rout[f], xs_b, so, w[f]);
+        for (int f = 0; f < 5; ++f) tf_struct_store_f32(w[f], rout[f], (int)xs_i, 0, (int)so, TF_STORE_AUX);
     }
 };
 
@@ -261,9 +287,8 @@ __device__ __forceinline__ void stream_row(const Ctx &T, const Gas &g, int r, fl
             godunov(qa[s].U, fa[s], fb[s], g.k, U2);  // the previous row is now Y-swept
             const Prim w2 = cons_to_prim(U2, g);
             const Q q2 = make_q<3>(U2, w2.v, w2.p, w2.rinv, g);
-            float fz[5], U3[5];
-            T.zflux(q2, fz, s);
-            T.zupdate(q2.U, fz, g.k, U3, s);
+            float U3[5];
+            T.zsweep(q2, g.k, U3, s);
             // U3 is complete here on every lane: no exec-mask change separates
             // the lane shifts above from their uses (which lets them fold into
             // the arithmetic); only the output lanes continue below.
@@ -359,10 +384,10 @@ __device__ __forceinline__ uint32_t fused_tile(const float *in_, float *out_, in
 // One workgroup of the line tiling (W waves, whole z-lines): returns the
 // wave-local max of the signal-speed bits of the cells this wave wrote.
 // `groups` x-column groups of nz lanes each (groups > 1 only when W == 1).
-template <int TX, int W>
+template <int TX>
 __device__ __forceinline__ uint32_t fused_line(const float *in_, float *out_, int nx, int ny, int nz, int open_bc,
-                                               const Gas &g_in, int groups, int xt, int yc, int ly, float *lq,
-                                               float *lf) {
+                                               const Gas &g_in, int W, int groups, int xt, int yc, int ly,
+                                               float *lq) {
     const Gas g{sgpr_f(g_in.gamma), sgpr_f(g_in.gm1), sgpr_f(g_in.inv_gm1), sgpr_f(g_in.k)};
     nx = __builtin_amdgcn_readfirstlane(nx);
     ny = __builtin_amdgcn_readfirstlane(ny);
@@ -372,15 +397,16 @@ __device__ __forceinline__ uint32_t fused_line(const float *in_, float *out_, in
     xt = __builtin_amdgcn_readfirstlane(xt);
     yc = __builtin_amdgcn_readfirstlane(yc);
     ly = __builtin_amdgcn_readfirstlane(ly);
-    LineCtx<TX, W> T;
+    LineCtx<TX> T;
+    T.lanes = 64 * W;
     const size_t nc = (size_t)nx * ny * nz;
     {
         const float *pin = sgpr_ptr(in_);
         float *pout = sgpr_ptr(out_);
 #pragma unroll
         for (int f = 0; f < 5; ++f) {
-            T.rin[f] = field_rsrc(pin + f * nc);
-            T.rout[f] = field_rsrc(pout + f * nc);
+            T.rin[f] = field_rsrc_indexed(pin + f * nc);
+            T.rout[f] = field_rsrc_indexed(pout + f * nc);
         }
     }
     const int gl = (int)threadIdx.x;  // 0 .. 64 W - 1
@@ -402,12 +428,12 @@ __device__ __forceinline__ uint32_t fused_line(const float *in_, float *out_, in
     const uint32_t plane_b = T.row_b * (uint32_t)ny;
     T.plane_b = plane_b;
     T.xs0 = xt * groups * TX + grp * TX;
-    T.xs_b = (uint32_t)min(T.xs0, nx - 1) * plane_b + 4u * (uint32_t)z;  // stores only where xs0 + s < nx
-    const uint32_t zb = 4u * (uint32_t)cell_index(z, nz, open_bc);
+    const uint32_t plane_c = (uint32_t)ny * (uint32_t)nz;  // cells per x-plane
+    T.xs_i = (uint32_t)min(T.xs0, nx - 1) * plane_c + (uint32_t)z;  // stores only where xs0 + s < nx
+    const uint32_t zc = (uint32_t)cell_index(z, nz, open_bc);
 #pragma unroll
-    for (int j = 0; j < TX + 2; ++j) T.xz[j] = (uint32_t)cell_index(T.xs0 - 1 + j, nx, open_bc) * plane_b + zb;
+    for (int j = 0; j < TX + 2; ++j) T.xi[j] = (uint32_t)cell_index(T.xs0 - 1 + j, nx, open_bc) * plane_c + zc;
     T.lq = lq;
-    T.lf = lf;
     const uint32_t yoff0 = (uint32_t)__builtin_amdgcn_readfirstlane(cell_index(T.y0 - 1, ny, open_bc)) * T.row_b;
     return stream_tile<TX>(T, g, yoff0);
 }
@@ -557,11 +583,10 @@ __global__ __launch_bounds__(kBlock) void k_step_sims_far(const SimDev *__restri
 template <int TX>
 constexpr int line_waves_per_simd() { return TX <= 2 ? 4 : (TX <= 4 ? 2 : 1); }
 
-template <int TX, int W>
-__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(line_waves_per_simd<TX>()))) void k_line_sims(
-    const SimDev *__restrict__ sims, const int *__restrict__ prefix, int nsims) {
-    __shared__ float lq[2 * 12 * 64 * W];
-    __shared__ float lf[2 * 5 * 64 * W];
+// Workgroup lb of a line class -> (simulation, x-tile, y-chunk) and the step.
+template <int TX>
+__device__ __forceinline__ void line_block(const SimDev *__restrict__ sims, const int *__restrict__ prefix, int nsims,
+                                           int W, float *lq) {
     constexpr int kXcd = 8;
     const int nblocks = prefix[nsims];
     const int q = (nblocks + kXcd - 1) / kXcd;
@@ -580,10 +605,31 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(line_wav
     Gas g{S.gamma, S.gm1, S.inv_gm1, C->k};
     const int local = lb - prefix[lo];
     const int xt = local % S.ntx, yc = local / S.ntx;
-    uint32_t v = fused_line<TX, W>(S.buf[par], S.buf[par ^ 1], S.nx, S.ny, S.nz, S.open_bc, g, S.groups, xt, yc,
-                                   S.ly, lq, lf);
+    uint32_t v = fused_line<TX>(S.buf[par], S.buf[par ^ 1], S.nx, S.ny, S.nz, S.open_bc, g, W, S.groups, xt, yc,
+                                S.ly, lq);
     v = wave_umax(v);
     if (lane_id() == 0) atomicMax(&C->max_bits[par ^ 1], v);
+}
+
+// Line tiling, W <= 4 waves per workgroup (W fixed): block b of a class of
+// simulations; simulation s owns blocks [prefix[s], prefix[s+1]) (XCD-aware
+// order). The line tiling keeps per-lane column offsets (TX + 2 VGPRs) on
+// top of the stream state, so it allows one resident wave fewer than the
+// halo tiling.
+template <int TX, int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(line_waves_per_simd<TX>()))) void k_line_sims(
+    const SimDev *__restrict__ sims, const int *__restrict__ prefix, int nsims) {
+    __shared__ float lq[2 * 13 * 64 * W];
+    line_block<TX>(sims, prefix, nsims, W, lq);
+}
+
+// Line tiling with 5 <= W <= MAXW waves per workgroup (W at run time): long
+// z-lines (up to 64 MAXW cells), one workgroup per CU.
+template <int TX, int MAXW>
+__global__ __launch_bounds__(64 * MAXW) void k_line_wide(const SimDev *__restrict__ sims, const int *__restrict__ prefix,
+                                                        int nsims, int W) {
+    __shared__ float lq[2 * 13 * 64 * MAXW];
+    line_block<TX>(sims, prefix, nsims, W, lq);
 }
 
 // ---------------------------------------------------------------- dt control
@@ -846,23 +892,31 @@ static hipError_t launch_line_w(int tx, const SimDev *sims, const int *prefix, i
     switch (tx) {
         case 2: hipLaunchKernelGGL((k_line_sims<2, W>), grid, block, 0, st, sims, prefix, nsims); break;
         case 4: hipLaunchKernelGGL((k_line_sims<4, W>), grid, block, 0, st, sims, prefix, nsims); break;
-        case 6: hipLaunchKernelGGL((k_line_sims<6, W>), grid, block, 0, st, sims, prefix, nsims); break;
-        case 8: hipLaunchKernelGGL((k_line_sims<8, W>), grid, block, 0, st, sims, prefix, nsims); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
+// Line classes: W <= 4 with tile width 2 or 4; 5 <= W <= 8 with 4 (two waves
+// per SIMD); 9 <= W <= 16 with 2 (four waves per SIMD).
 hipError_t launch_line_sims(int tx, int w, const SimDev *sims, const int *prefix, int nsims, int nblocks,
                             hipStream_t st) {
     if (nsims <= 0 || nblocks <= 0) return hipSuccess;
+    const dim3 grid((unsigned)(((nblocks + 7) / 8) * 8)), block(64 * w);
     switch (w) {
         case 1: return launch_line_w<1>(tx, sims, prefix, nsims, nblocks, st);
         case 2: return launch_line_w<2>(tx, sims, prefix, nsims, nblocks, st);
         case 3: return launch_line_w<3>(tx, sims, prefix, nsims, nblocks, st);
         case 4: return launch_line_w<4>(tx, sims, prefix, nsims, nblocks, st);
-        default: return hipErrorInvalidValue;
+        default: break;
     }
+    if (w <= 8 && tx == 4)
+        hipLaunchKernelGGL((k_line_wide<4, 8>), grid, block, 0, st, sims, prefix, nsims, w);
+    else if (w <= 16 && tx == 2)
+        hipLaunchKernelGGL((k_line_wide<2, 16>), grid, block, 0, st, sims, prefix, nsims, w);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
 }
 
 hipError_t launch_step_sims(int tx, const SimDev *sims, const int *prefix, int nsims, int total_waves,
