@@ -4,12 +4,16 @@ Workload (one step = CFL dt + the X -> Y -> Z split HLL/Godunov update, the
 body of time_integration.py:63-83): Kelvin-Helmholtz shear layer on a
 1024^3 periodic float32 grid, gamma = 1.4, dx = 1/1024, 50 timed steps.
 The state stays resident in HBM; each step is one fused stencil launch (dt
-reduction fused into its epilogue) plus a one-thread dt update.
+reduction fused into its epilogue) plus a one-thread dt update. The tiling
+is the one the loop tuner measures fastest (for 1024^3: whole z-lines, 16
+waves per workgroup).
 
 Prints one JSON line: cell-updates/s, the HBM roofline of the stencil kernel
 (40 algorithmic bytes per cell-update: 5 float32 read + 5 written), and the
 CPU restatement of the reference algorithm (C, float32, OpenMP) timed on a
-bounded z-slab of the same workload.
+bounded z-slab of the same workload; further legs: the non-cubic grid, the
+reference's own harness against its published GH200 times, the uneven batch
+and the served job stream against the plain path.
 """
 
 from __future__ import annotations
@@ -149,19 +153,19 @@ def harness_leg(dev, sizes):
     return out
 
 
-def measured_traffic(nx, ny, nz, tile):
+def measured_traffic(nx, ny, nz, kernel, tile):
     """HBM bytes per launch of the fused step from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary (scaled by a plain copy's counter reading),
-    when it was taken on this workload and tiling; else None."""
+    when it was taken on this workload, kernel and tiling; else None."""
     path = os.path.join(ROOT, "profiles", "fused_step_kh1024_hbm_traffic.md")
     if (nx, ny, nz) != (1024, 1024, 1024) or not os.path.exists(path):
         return None, None
     fields = {}
     for line in open(path):
-        if ":" in line and line.split(":")[0] in ("traffic_bytes_per_launch", "tile"):
+        if ":" in line and line.split(":")[0] in ("traffic_bytes_per_launch", "tile", "kernel"):
             k, v = line.split(":", 1)
             fields[k.strip()] = v.strip()
-    if fields.get("tile") != f"{tile[0]},{tile[1]}":
+    if fields.get("tile") != f"{tile[0]},{tile[1]}" or fields.get("kernel") != kernel:
         return None, None
     return int(fields["traffic_bytes_per_launch"]), os.path.relpath(path, ROOT)
 
@@ -219,17 +223,26 @@ def serve_leg(dev, njobs, rates):
     return out
 
 
+def plan_kernel(tx, waves_per_line):
+    """Name of the stencil kernel a plan runs for a simulation's tiling."""
+    if waves_per_line == 0:
+        return f"tf::k_step_sims<{tx}>"
+    if waves_per_line <= 4:
+        return f"tf::k_line_sims<{tx}, {waves_per_line}>"
+    return "tf::k_line_wide<4, 8>" if tx == 4 else "tf::k_line_wide<2, 16>"
+
+
 def measure_grid(dev, nx, ny, nz, steps, warmup, tile, no_tune, calibrate):
-    """KH on nx x ny x nz resident in HBM: tiling (forced, autotuned or the
-    default), `warmup` steps, then `steps` timed steps (wall clock and HIP
-    events around the stencil launches)."""
+    """KH on nx x ny x nz resident in HBM: tiling (forced, measured by the
+    loop tuner, or the default), `warmup` steps, then `steps` timed steps
+    (wall clock and HIP events around the stencil launches)."""
     import ctypes
 
     import torch
 
-    import tinyfluids_amd as tf
     from tinyfluids_amd import _lib, ics
     from tinyfluids_amd.batch import Batch, Simulation
+    from tinyfluids_amd.time_integration import autotune
 
     dx = 1.0 / ny
     gamma = ics.GAMMA_KH
@@ -239,28 +252,22 @@ def measure_grid(dev, nx, ny, nz, steps, warmup, tile, no_tune, calibrate):
     buf1 = torch.empty_like(buf0)
     print(f"bench: KH {nx}x{ny}x{nz} fp32, state {buf0.numel() * 4 / 1e9:.2f} GB", file=sys.stderr, flush=True)
     lib = _lib.load()
-    tx, ly = ctypes.c_int(0), ctypes.c_int(0)
+    tx = ty = 0
     tuned_ms = None
     if tile:
-        tx.value, ly.value = (int(v) for v in tile.split(","))
-        _lib.check(lib.tf_set_tiling(nx, ny, nz, tx.value, ly.value))
+        tx, ty = (int(v) for v in tile.split(","))
+        _lib.check(lib.tf_set_tiling(nx, ny, nz, tx, ty))
     elif not no_tune:
-        ws = tf.fluid.workspace(nx, ny, nz, dev)
-        ms = ctypes.c_float(0)
-        _lib.check(lib.tf_autotune_evolve(buf0.data_ptr(), buf1.data_ptr(), nx, ny, nz, dx, gamma, ws.data_ptr(),
-                                          ctypes.byref(tx), ctypes.byref(ly), ctypes.byref(ms),
-                                          tf.fluid._stream(dev)))
-        tuned_ms = ms.value
-        del ws
-    else:
-        _lib.check(lib.tf_get_tiling(nx, ny, nz, ctypes.byref(tx), ctypes.byref(ly)))
-    print(f"bench: tiling tile_x={tx.value} tile_y={ly.value} (tuned {tuned_ms} ms)", file=sys.stderr, flush=True)
+        _, _, tuned_ms = autotune(buf0, dx, gamma, False, buf1)
     if calibrate:
         buf1.copy_(buf0)
         torch.cuda.synchronize()
         print(f"bench: calibration copy of {buf0.numel() * 4} bytes", file=sys.stderr, flush=True)
     sim = Simulation(buf0, dx, gamma, buffers=(buf0, buf1))
-    batch = Batch([sim], tile_x=tx.value, tile_y=ly.value)
+    batch = Batch([sim], tile_x=tx, tile_y=ty)
+    ptx, pty, pw = batch.tiling(0)
+    print(f"bench: plan tile_x={ptx} tile_y={pty} waves/z-line={pw} (tuned {tuned_ms} ms)", file=sys.stderr,
+          flush=True)
     sim.reset()
     batch.step(warmup)
     torch.cuda.synchronize()
@@ -273,7 +280,8 @@ def measure_grid(dev, nx, ny, nz, steps, warmup, tile, no_tune, calibrate):
     batch.close()
     del buf0, buf1, sim, batch
     torch.cuda.empty_cache()
-    return {"tile_x": tx.value, "tile_y": ly.value, "status": st, "seconds": el, "stencil_ms": stencil_ms}
+    return {"tile_x": ptx, "tile_y": pty, "waves_per_line": pw, "kernel": plan_kernel(ptx, pw), "status": st,
+            "seconds": el, "stencil_ms": stencil_ms}
 
 
 def main():
@@ -288,9 +296,10 @@ def main():
     dev = torch.device("cuda:0")
     m = measure_grid(dev, nx, ny, nz, args.steps, args.warmup, args.tile, args.no_tune, args.calibrate_copy)
     tx, ly, st, el, stencil_ms = m["tile_x"], m["tile_y"], m["status"], m["seconds"], m["stencil_ms"]
+    kernel = m["kernel"]
     cells = nx * ny * nz
     value = cells * args.steps / el
-    traffic, traffic_src = measured_traffic(nx, ny, nz, (tx, ly))
+    traffic, traffic_src = measured_traffic(nx, ny, nz, kernel, (tx, ly))
     kernel_s = stencil_ms / 1e3 / args.steps
     achieved = BYTES_PER_CELL * cells / kernel_s / 1e9
     rec = {
@@ -308,10 +317,11 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"Kelvin-Helmholtz shear layer {nx}x{ny}x{nz} periodic, gamma 1.4, dx 1/{ny}, "
                                f"CFL 0.4, fused split HLL/Godunov step with dt",
-                   "grid": [nx, ny, nz], "tile_x": tx, "tile_y": ly, "parallelism": "single GPU"},
+                   "grid": [nx, ny, nz], "tile_x": tx, "tile_y": ly, "waves_per_z_line": m["waves_per_line"],
+                   "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
-                     "kernel": f"tf::k_step_sims<{tx}>", "kernel_ms": kernel_s * 1e3,
+                     "kernel": kernel, "kernel_ms": kernel_s * 1e3,
                      "algorithmic_bytes_per_launch": BYTES_PER_CELL * cells},
         "sim_time": st.t,
     }
@@ -322,7 +332,7 @@ def main():
         rec["shapes"] = [{"grid": [1536, 512, 512], "workload": "Kelvin-Helmholtz, gamma 1.4, dx 1/512, 20 steps",
                           "value": c2 * 20 / m2["seconds"], "ms_per_step": m2["seconds"] * 1e3 / 20,
                           "kernel_ms": k2 * 1e3, "frac": BYTES_PER_CELL * c2 / k2 / 1e9 / HBM_PEAK_GBS,
-                          "tile_x": m2["tile_x"], "tile_y": m2["tile_y"]}]
+                          "tile_x": m2["tile_x"], "tile_y": m2["tile_y"], "kernel": m2["kernel"]}]
         print(f"bench: shapes {rec['shapes']}", file=sys.stderr, flush=True)
     torch.cuda.empty_cache()
     if not args.no_harness:

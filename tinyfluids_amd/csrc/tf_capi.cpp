@@ -870,7 +870,8 @@ int tf_autotune_loop(float *state, float *scratch, int nx, int ny, int nz, doubl
     const double cells = (double)nx * ny * nz;
     const bool big = cells > 67108864.0;
     std::vector<int> txs = big ? std::vector<int>{4, 8} : std::vector<int>{2, 4, 8};
-    std::vector<int> lys = big ? std::vector<int>{16, 32, 64} : std::vector<int>{0, 1, 2, 4, 8, 16, 32, 64};
+    std::vector<int> lys = big ? std::vector<int>{16, 32, 64, 128, 256}
+                               : std::vector<int>{0, 1, 2, 4, 8, 16, 32, 64, 128};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     TF_HIP(hipEventCreate(&e0));
     TF_HIP(hipEventCreate(&e1));
