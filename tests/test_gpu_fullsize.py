@@ -114,3 +114,32 @@ def test_blast_noise_512_against_float64_fixture():
     assert np.all(np.abs(totals[:, 4] - ref_tot[:, 4]) <= 1e-5 * ref_tot[0, 4])
     assert np.all(np.abs(totals[:, 1:4]) <= 1e-6 * ref_tot[0, 0])
     assert abs(st.t - float(np.sum(g["dts"]))) <= 1e-5 * float(np.sum(g["dts"]))
+
+
+def test_single_step_path_equals_plan_at_2_30_cells():
+    """On a 2^30-cell grid the single-step entry point (62-cell z-tiles, raw
+    buffer offsets, the last (x, y) row in the 64-bit-address kernel) and the
+    loop's plan (whole z-lines, indexed buffer access) give the same step bit
+    for bit, including the cells next to the last dword of each field."""
+    import tinyfluids_amd as tf
+    from tinyfluids_amd.batch import Batch, Simulation
+    nx = ny = nz = 1024
+    dx = 1.0 / ny
+    rng = np.random.default_rng(5)
+    plane = ics.kelvin_helmholtz_plane(nx, ny, dx)
+    buf0 = torch.from_numpy(plane).to(DEV)[:, :, :, None].expand(5, nx, ny, nz).contiguous()
+    # break the z-invariance near the last cell so that a wrong value there shows
+    corner = torch.from_numpy(rng.uniform(0.9, 1.1, (5, 3, 3, 3)).astype(np.float32)).to(DEV)
+    buf0[:, -3:, -3:, -3:] *= corner
+    buf0[:, :2, :2, :2] *= corner[:, :2, :2, :2]
+    dt = tf._cfl_time_step(buf0, dx, ics.GAMMA_KH)
+    direct = tf._evolve_state(buf0, dx, dt, ics.GAMMA_KH)
+    buf1 = torch.empty_like(buf0)
+    sim = Simulation(buf0, dx, ics.GAMMA_KH, max_steps=1, buffers=(buf0, buf1))
+    b = Batch([sim])
+    assert b.tiling(0)[2] == 16  # whole z-lines of 16 waves
+    sim.reset()
+    b.step(1)
+    st = b.status()[0]
+    assert st.n_iter == 1
+    assert torch.equal(sim.state(st), direct)
