@@ -429,6 +429,14 @@ struct tf_batch {
     hipGraphExec_t exec = nullptr;
     hipStream_t capture_stream = nullptr;
     int graph_steps = 0;
+    // Concurrent launches of the tiling classes of one step: side streams
+    // (default priority) forked from and joined back into the caller's stream
+    // with events; created on first use when a plan has several classes.
+    static constexpr int kSide = 3;
+    hipStream_t side[kSide] = {};
+    hipEvent_t fork_ev = nullptr;
+    hipEvent_t join_ev[kSide] = {};
+    int nlaunch = 0;  // non-empty stencil launches per step (classes + far rows)
 };
 
 static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
@@ -631,6 +639,8 @@ int tf_batch_create(tf_batch **out, const tf_sim_desc *sims, int nsims, void *wo
         if (far.size() > before) b->far_max_ntz = std::max(b->far_max_ntz, b->h_sims[k].ntz);
     }
     b->nfar = (int)far.size();
+    b->nlaunch = (b->nfar > 0 ? 1 : 0);
+    for (int c = 0; c < kClasses; ++c) b->nlaunch += b->cls[c].count > 0 ? 1 : 0;
     hipStream_t st = S_(stream);
     hipError_t e = hipMemcpyAsync(b->d_sims, b->h_sims.data(), nsims * sizeof(SimDev), hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
@@ -666,27 +676,70 @@ int tf_batch_destroy(tf_batch *b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
+    for (int i = 0; i < tf_batch::kSide; ++i) {
+        if (b->side[i]) (void)hipStreamDestroy(b->side[i]);
+        if (b->join_ev[i]) (void)hipEventDestroy(b->join_ev[i]);
+    }
+    if (b->fork_ev) (void)hipEventDestroy(b->fork_ev);
     delete b;
     return TF_OK;
 }
 
 // One batched stencil step: the halo-tiled class, each line-tiled class, then
 // the far-row tiles (all before the dt update).
-static hipError_t launch_stencils(tf_batch *b, hipStream_t st) {
-    const SimClass &H = b->cls[0];
-    hipError_t e = hipSuccess;
-    if (H.count > 0) e = launch_step_sims(b->tx, b->d_sims + H.begin, H.d_prefix, H.count, H.total, st);
-    for (int c = 1; c < kClasses && e == hipSuccess; ++c) {
-        const SimClass &L = b->cls[c];
-        if (L.count > 0) e = launch_line_sims(L.tx, c, b->d_sims + L.begin, L.d_prefix, L.count, L.total, st);
+// The k-th non-empty stencil launch of a step (tiling classes in order, then
+// the far rows) on stream st.
+static hipError_t launch_stencil_part(tf_batch *b, int k, hipStream_t st) {
+    int i = 0;
+    for (int c = 0; c < kClasses; ++c) {
+        const SimClass &C = b->cls[c];
+        if (C.count <= 0) continue;
+        if (i++ != k) continue;
+        if (c == 0) return launch_step_sims(C.tx, b->d_sims + C.begin, C.d_prefix, C.count, C.total, st);
+        return launch_line_sims(C.tx, c, b->d_sims + C.begin, C.d_prefix, C.count, C.total, st);
     }
-    if (e == hipSuccess) e = launch_step_sims_far(b->tx, b->d_sims, b->d_far, b->nfar, b->far_max_ntz, st);
+    if (b->nfar > 0 && i == k) return launch_step_sims_far(b->tx, b->d_sims, b->d_far, b->nfar, b->far_max_ntz, st);
+    return hipSuccess;
+}
+
+// One batched stencil step: every tiling class and the far-row tiles, in
+// order on st, or, with `concurrent` and more than one launch, launch 0 on st
+// and the others on side streams forked from st and joined back (each
+// launch's tail then overlaps the others instead of draining the device
+// between them). Captured graphs use the concurrent form, where the fork and
+// join cost nothing at replay; eager steps use the ordered form, whose host
+// cost per step is lower.
+static hipError_t launch_stencils(tf_batch *b, hipStream_t st, bool concurrent) {
+    if (!concurrent || b->nlaunch <= 1) {
+        hipError_t e = hipSuccess;
+        for (int k = 0; k < std::max(1, b->nlaunch) && e == hipSuccess; ++k) e = launch_stencil_part(b, k, st);
+        return e;
+    }
+    hipError_t e = hipSuccess;
+    if (!b->fork_ev) {
+        e = hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming);
+        for (int i = 0; i < tf_batch::kSide && e == hipSuccess; ++i) {
+            e = hipStreamCreateWithFlags(&b->side[i], hipStreamNonBlocking);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&b->join_ev[i], hipEventDisableTiming);
+        }
+        if (e != hipSuccess) return e;
+    }
+    const int nside = std::min(b->nlaunch - 1, (int)tf_batch::kSide);
+    e = hipEventRecord(b->fork_ev, st);
+    for (int i = 0; i < nside && e == hipSuccess; ++i) e = hipStreamWaitEvent(b->side[i], b->fork_ev, 0);
+    // launch k > 0 goes to side stream (k - 1) % nside; launch 0 to st
+    for (int k = 1; k < b->nlaunch && e == hipSuccess; ++k) e = launch_stencil_part(b, k, b->side[(k - 1) % nside]);
+    if (e == hipSuccess) e = launch_stencil_part(b, 0, st);
+    for (int i = 0; i < nside && e == hipSuccess; ++i) {
+        e = hipEventRecord(b->join_ev[i], b->side[i]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(st, b->join_ev[i], 0);
+    }
     return e;
 }
 
-static int enqueue_steps(tf_batch *b, int steps, hipStream_t st) {
+static int enqueue_steps(tf_batch *b, int steps, hipStream_t st, bool concurrent = false) {
     for (int i = 0; i < steps; ++i) {
-        TF_HIP(launch_stencils(b, st));
+        TF_HIP(launch_stencils(b, st, concurrent));
         TF_HIP(launch_finalize_sims(b->d_sims, b->nsims, st));
     }
     return TF_OK;
@@ -712,7 +765,7 @@ int tf_batch_step_graph(tf_batch *b, int steps, void *stream) {
         }
         if (!b->capture_stream) TF_HIP(hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking));
         TF_HIP(hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal));
-        const int rc = enqueue_steps(b, steps, b->capture_stream);
+        const int rc = enqueue_steps(b, steps, b->capture_stream, true);
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(b->capture_stream, &g);
         if (rc != TF_OK) {
@@ -742,7 +795,7 @@ int tf_batch_step_timed(tf_batch *b, int steps, float *stencil_ms, void *stream)
     }
     for (int i = 0; i < steps && rc == TF_OK; ++i) {
         hipError_t e = hipEventRecord(ev[2 * i], st);
-        if (e == hipSuccess) e = launch_stencils(b, st);
+        if (e == hipSuccess) e = launch_stencils(b, st, false);
         if (e == hipSuccess) e = hipEventRecord(ev[2 * i + 1], st);
         if (e == hipSuccess) e = launch_finalize_sims(b->d_sims, b->nsims, st);
         if (e != hipSuccess) rc = fail(TF_ERR_HIP, "timed step: %s", hipGetErrorString(e));
@@ -892,9 +945,9 @@ int tf_autotune_loop(float *state, float *scratch, int nx, int ny, int nz, doubl
             if (std::find(seen.begin(), seen.end(), key) != seen.end()) continue;
             seen.push_back(key);
             const int reps = big ? 2 : 5;
-            hipError_t e = launch_stencils(plan.b, st);
+            hipError_t e = launch_stencils(plan.b, st, false);
             if (e == hipSuccess) e = hipEventRecord(e0, st);
-            for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_stencils(plan.b, st);
+            for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_stencils(plan.b, st, false);
             if (e == hipSuccess) e = hipEventRecord(e1, st);
             if (e == hipSuccess) e = hipEventSynchronize(e1);
             float ms = 0.f;
