@@ -55,3 +55,56 @@ def test_setup_ics_matches_reference_harness():
     assert s[0, 16, 16, 16] == 1.0 and s[0, 0, 0, 0] == np.float32(0.125)
     assert s[4, 14, 14, 14] == 1.0 and s[4, 13, 14, 14] == np.float32(0.1)
     assert int((s[0] == 1.0).sum()) == 4 ** 3
+
+
+def _jobs(spec_list):
+    from tinyfluids_amd.engine import JobRecord
+    return [JobRecord(i, s, s.arrival) for i, s in enumerate(spec_list)]
+
+
+def _spec(shape, steps, arrival):
+    return ics.SimSpec(tuple(shape), ics.GAMMA_BLAST, 1.0 / shape[0], steps, arrival=arrival)
+
+
+def test_server_groups_by_size():
+    from tinyfluids_amd.engine import Server
+    assert Server.group_of(_spec((64, 64, 64), 10, 0.0)) == 0           # 2^18 cells
+    assert Server.group_of(_spec((128, 128, 64), 10, 0.0)) == 1         # 2^20
+    assert Server.group_of(_spec((256, 256, 256), 10, 0.0)) == 2        # 2^24
+    assert Server.group_of(_spec((256, 128, 128), 10, 0.0)) == 1        # 2^22
+
+
+def test_server_batch_choice_oldest_then_shortest_work():
+    """The oldest job is always in the batch; the rest come shortest remaining
+    work first (policy "shortest") or in arrival order ("fcfs"), until the
+    batch holds fill_cells cells."""
+    from tinyfluids_amd.engine import Server
+    specs = [_spec((128, 128, 128), 1000, 0.0),   # oldest, 2M cells
+             _spec((64, 64, 64), 2000, 0.1),      # 262k cells, long
+             _spec((64, 64, 64), 10, 0.2),        # 262k cells, short
+             _spec((128, 128, 128), 20, 0.3)]     # 2M cells, short-ish
+    jobs = _jobs(specs)
+    srv = Server(1 << 40, fill_cells=(1 << 21) + (1 << 18) + 1)
+    chosen = srv._select([0, 1, 2, 3], jobs)
+    # remaining work: job2 2.6M < job3 42M < job1 524M -> fill: 0 (2M), 2 (262k), then 3 overflows the fill
+    assert chosen == [0, 2, 3]
+    fcfs = Server(1 << 40, fill_cells=(1 << 21) + (1 << 18) + 1, policy="fcfs")
+    assert fcfs._select([0, 1, 2, 3], jobs) == [0, 1, 2]
+    jobs[0].steps_done = 999  # the oldest stays first whatever its remaining work
+    assert srv._select([3, 2, 0], jobs)[0] == 0
+    with pytest.raises(ValueError):
+        Server(1 << 30, policy="random")
+
+
+def test_server_tick_length_follows_step_cost():
+    from tinyfluids_amd.engine import Server
+    small = _jobs([_spec((32, 32, 32), 5000, 0.0)])
+    big = _jobs([_spec((256, 256, 256), 5000, 0.0)])
+    srv = Server(1 << 40, tick_ms=4.0)
+    n_small, n_big = srv._tick_steps([0], small), srv._tick_steps([0], big)
+    assert n_small > 4 * n_big >= 4          # small grids take many more steps per tick
+    assert n_small <= 1024
+    nearly_done = _jobs([_spec((32, 32, 32), 5000, 0.0)])
+    nearly_done[0].steps_done = 4997
+    assert srv._tick_steps([0], nearly_done) == 3  # no more steps than are left
+    assert Server(1 << 40, steps_per_tick=7)._tick_steps([0], big) == 7
