@@ -83,7 +83,11 @@ int tf_evolve_state_along_axis(const float *in, float *out, int nx, int ny, int 
  * handle_boundaries] then the X, Y and Z sweeps, fused into one pass over
  * HBM. dt is a device pointer to a float64. in and out must not alias.
  * If max_speed_out is not NULL, the maximum per-cell |v_d| + c of the new
- * state is max-accumulated into it (as float bits; clear it first). */
+ * state is max-accumulated into it (as float bits; clear it first).
+ * With a workspace and nz <= 1024 the step runs whole z-lines per workgroup
+ * and keeps its table and control block in the workspace header: calls that
+ * share a workspace must be ordered on one stream. Without a workspace it
+ * runs 62-cell z-tiles. */
 int tf_evolve_state(const float *in, float *out, int nx, int ny, int nz, double grid_spacing, const double *dt,
                     double gamma, int handle_boundaries, unsigned int *max_speed_out, void *workspace, void *stream);
 

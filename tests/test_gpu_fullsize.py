@@ -117,10 +117,11 @@ def test_blast_noise_512_against_float64_fixture():
 
 
 def test_single_step_path_equals_plan_at_2_30_cells():
-    """On a 2^30-cell grid the single-step entry point (62-cell z-tiles, raw
-    buffer offsets, the last (x, y) row in the 64-bit-address kernel) and the
-    loop's plan (whole z-lines, indexed buffer access) give the same step bit
-    for bit, including the cells next to the last dword of each field."""
+    """On a 2^30-cell grid the single-step entry point with 62-cell z-tiles
+    (raw buffer offsets, the last (x, y) row in the 64-bit-address kernel),
+    the same entry point with whole z-lines, and the loop's plan (whole
+    z-lines, indexed buffer access) give the same step bit for bit, including
+    the cells next to the last dword of each field."""
     import tinyfluids_amd as tf
     from tinyfluids_amd.batch import Batch, Simulation
     nx = ny = nz = 1024
@@ -132,8 +133,15 @@ def test_single_step_path_equals_plan_at_2_30_cells():
     corner = torch.from_numpy(rng.uniform(0.9, 1.1, (5, 3, 3, 3)).astype(np.float32)).to(DEV)
     buf0[:, -3:, -3:, -3:] *= corner
     buf0[:, :2, :2, :2] *= corner[:, :2, :2, :2]
+    import tinyfluids_amd._lib as L
     dt = tf._cfl_time_step(buf0, dx, ics.GAMMA_KH)
-    direct = tf._evolve_state(buf0, dx, dt, ics.GAMMA_KH)
+    lib = L.load()
+    try:
+        lib.tf_set_line_mode(0)  # the single-step path with 62-cell z-tiles
+        direct = tf._evolve_state(buf0, dx, dt, ics.GAMMA_KH)
+    finally:
+        lib.tf_set_line_mode(1)
+    assert torch.equal(tf._evolve_state(buf0, dx, dt, ics.GAMMA_KH), direct)  # whole lines, single step
     buf1 = torch.empty_like(buf0)
     sim = Simulation(buf0, dx, ics.GAMMA_KH, max_steps=1, buffers=(buf0, buf1))
     b = Batch([sim])

@@ -29,6 +29,9 @@ hipError_t launch_step_sims_far(int tx, const SimDev *sims, const FarTile *far, 
 hipError_t launch_line_sims(int tx, int w, const SimDev *sims, const int *prefix, int nsims, int nblocks,
                             hipStream_t st);
 hipError_t launch_finalize_sims(const SimDev *sims, int nsims, hipStream_t st);
+hipError_t launch_direct_setup(const SimDev &S, SimDev *dst, int *prefix, int nblocks, const double *dt, double dx,
+                               hipStream_t st);
+hipError_t launch_direct_bits(const Ctrl *C, uint32_t *bits_out, hipStream_t st);
 hipError_t launch_finalize_one(Ctrl *C, int first, hipStream_t st);
 hipError_t launch_ctrl_reset(Ctrl *C, double dx, double c_cfl, double t_final, int64_t max_steps, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, int n, hipStream_t st);
@@ -220,6 +223,33 @@ int check_open_bc(int open_bc, int nx, int ny, int nz) {
     return TF_OK;
 }
 
+constexpr int kMaxLineW = 16;          // whole z-lines of up to 1024 cells per workgroup
+constexpr int kClasses = kMaxLineW + 1;  // halo tiling + line tiling with W = 1..16
+std::atomic<int> g_line_mode{1};
+
+// Line tiling of a z-line of nz cells: w waves per workgroup, and with
+// w == 1 as many x-column groups as whole lines fit in a wave.
+void line_shape(int nz, int &w, int &groups) {
+    w = (nz + 63) / 64;
+    groups = w == 1 ? std::max(1, 64 / nz) : 1;
+}
+double halo_lane_use(int nz) { return (double)nz / (64.0 * ((nz + kZT - 1) / kZT)); }
+// Tile width of a line class: at most 4 (registers for two waves per SIMD),
+// and 2 above 8 waves per workgroup (four waves per SIMD).
+int line_tx(int w, int tx) { return w > 8 ? 2 : std::min(tx, 4); }
+
+double line_lane_use(int nz) {
+    int w, g;
+    line_shape(nz, w, g);
+    return (double)(g * nz) / (64.0 * w);
+}
+
+// Whole z-lines whenever they fit a workgroup and use the lanes at least as
+// well as 62-cell z-tiles: every row segment then starts at its line's start
+// instead of one cell before a 62-cell tile.
+bool use_line_tiling(int nz) {
+    return g_line_mode.load() != 0 && nz <= 64 * kMaxLineW && line_lane_use(nz) >= halo_lane_use(nz) - 1e-9;
+}
 }  // namespace
 
 extern "C" {
@@ -291,7 +321,6 @@ int tf_evolve_state_along_axis(const float *in, float *out, int nx, int ny, int 
 int tf_evolve_state(const float *in, float *out, int nx, int ny, int nz, double grid_spacing, const double *dt,
                     double gamma, int handle_boundaries, unsigned int *max_speed_out, void *workspace,
                     void *stream) {
-    (void)workspace;
     if (int e = check_dims(nx, ny, nz)) return e;
     if (int e = check_fused_size(nx, ny, nz)) return e;
     TF_REQUIRE(in && out && dt, "null pointer");
@@ -299,6 +328,32 @@ int tf_evolve_state(const float *in, float *out, int nx, int ny, int nz, double 
     if (int e = check_open_bc(handle_boundaries, nx, ny, nz)) return e;
     int tx, ly;
     tiling_for(nx, ny, nz, tx, ly);
+    if (workspace && use_line_tiling(nz)) {
+        // Whole z-lines, as in loops and batches: the table and the control
+        // block (step 0, k from dt) are written on the stream by a setup
+        // kernel, so the call stays asynchronous.
+        auto *H = reinterpret_cast<WsHeader *>(workspace);
+        int w, groups;
+        line_shape(nz, w, groups);
+        const int txl = line_tx(w, tx);
+        int lx, ll;
+        loop_tiling_for(nx, ny, nz, lx, ll);
+        SimDev S = make_simdev(nx, ny, nz, handle_boundaries, gamma, txl, ll > 0 ? ll : ly);
+        S.buf[0] = const_cast<float *>(in);
+        S.buf[1] = out;
+        S.ctrl = &H->ctrl;
+        S.line_w = w;
+        S.groups = groups;
+        S.ntx = (nx + groups * txl - 1) / (groups * txl);
+        S.ntz = 1;
+        S.waves = S.ntx * S.nyc * w;
+        const int nblocks = S.ntx * S.nyc;
+        hipStream_t st = S_(stream);
+        TF_HIP(launch_direct_setup(S, &H->sim, H->prefix, nblocks, dt, grid_spacing, st));
+        TF_HIP(launch_line_sims(txl, w, &H->sim, H->prefix, 1, nblocks, st));
+        if (max_speed_out) TF_HIP(launch_direct_bits(&H->ctrl, max_speed_out, st));
+        return TF_OK;
+    }
     const SimDev S = make_simdev(nx, ny, nz, handle_boundaries, gamma, tx, ly);
     TF_HIP(launch_step_direct(tx, S, in, out, dt, grid_spacing, max_speed_out, S_(stream)));
     TF_HIP(launch_step_direct_far(tx, S, far_list(S, tx), in, out, dt, grid_spacing, max_speed_out, S_(stream)));
@@ -401,8 +456,6 @@ int tf_autotune_evolve(const float *in, float *out, int nx, int ny, int nz, doub
 // ------------------------------------------------------------------ batched engine
 // A class of simulations launched together: halo tiling (w == 0, prefix over
 // waves) or line tiling with w waves per workgroup (prefix over workgroups).
-constexpr int kMaxLineW = 16;          // whole z-lines of up to 1024 cells per workgroup
-constexpr int kClasses = kMaxLineW + 1;  // halo tiling + line tiling with W = 1..16
 
 struct SimClass {
     int w = 0;
@@ -448,25 +501,6 @@ size_t tf_batch_workspace_bytes(int nsims) {
            align256((size_t)nsims * kMaxFarPerSim * kFarSplit * sizeof(FarTile));
 }
 
-namespace {
-std::atomic<int> g_line_mode{1};
-
-// Line tiling of a z-line of nz cells: w waves per workgroup, and with
-// w == 1 as many x-column groups as whole lines fit in a wave.
-void line_shape(int nz, int &w, int &groups) {
-    w = (nz + 63) / 64;
-    groups = w == 1 ? std::max(1, 64 / nz) : 1;
-}
-double halo_lane_use(int nz) { return (double)nz / (64.0 * ((nz + kZT - 1) / kZT)); }
-// Tile width of a line class: at most 4 (registers for two waves per SIMD),
-// and 2 above 8 waves per workgroup (four waves per SIMD).
-int line_tx(int w, int tx) { return w > 8 ? 2 : std::min(tx, 4); }
-double line_lane_use(int nz) {
-    int w, g;
-    line_shape(nz, w, g);
-    return (double)(g * nz) / (64.0 * w);
-}
-}  // namespace
 
 int tf_set_line_mode(int enabled) {
     g_line_mode.store(enabled ? 1 : 0);
@@ -531,7 +565,6 @@ int tf_batch_create(tf_batch **out, const tf_sim_desc *sims, int nsims, void *wo
         if (int e = check_open_bc(d.handle_boundaries, d.nx, d.ny, d.nz)) return e;
         TF_REQUIRE(d.buf0 && d.buf1 && d.ctrl, "null device pointer in sim %d", i);
     }
-    const bool line_on = g_line_mode.load() != 0;
     auto build = [&](int ly_cap, std::vector<SimDev> &tab) {
         long total = 0;
         for (int i = 0; i < nsims; ++i) {
@@ -542,10 +575,7 @@ int tf_batch_create(tf_batch **out, const tf_sim_desc *sims, int nsims, void *wo
             S.buf[0] = d.buf0;
             S.buf[1] = d.buf1;
             S.ctrl = reinterpret_cast<Ctrl *>(d.ctrl);
-            // Whole z-lines whenever they fit a workgroup and use the lanes at
-            // least as well: every row segment then starts at its line's
-            // start instead of one cell before a 62-cell tile.
-            if (line_on && d.nz <= 64 * kMaxLineW && line_lane_use(d.nz) >= halo_lane_use(d.nz) - 1e-9) {
+            if (use_line_tiling(d.nz)) {
                 int w, groups;
                 line_shape(d.nz, w, groups);
                 const int txl = line_tx(w, tx);

@@ -656,6 +656,28 @@ __global__ void k_finalize_sims(const SimDev *__restrict__ sims, int nsims) {
     if (i < nsims) finalize_ctrl(sims[i].ctrl, 0);
 }
 
+// Table and control block of a single explicit-dt step through the
+// whole-line kernels: the simulation (buf[0] = input, buf[1] = output) at
+// step 0 with k = float(dt / dx), one class of `nblocks` workgroups.
+__global__ void k_direct_setup(SimDev S, SimDev *dst, int *prefix, int nblocks, const double *dt, double dx) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    *dst = S;
+    prefix[0] = 0;
+    prefix[1] = nblocks;
+    Ctrl *C = S.ctrl;
+    C->n_iter = 0;
+    C->active = 1;
+    C->nonfinite = 0;
+    C->max_bits[0] = 0u;
+    C->max_bits[1] = 0u;
+    C->k = (float)(*dt / dx);
+}
+
+// Max-accumulates the signal speed of the state a direct step wrote.
+__global__ void k_direct_bits(const Ctrl *C, uint32_t *bits_out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) atomicMax(bits_out, C->max_bits[1]);
+}
+
 __global__ void k_finalize_one(Ctrl *C, int first) {
     if (threadIdx.x == 0 && blockIdx.x == 0) finalize_ctrl(C, first);
 }
@@ -930,6 +952,17 @@ hipError_t launch_step_sims(int tx, const SimDev *sims, const int *prefix, int n
         case 8: hipLaunchKernelGGL(k_step_sims<8>, grid, block, 0, st, sims, prefix, nsims); break;
         default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_direct_setup(const SimDev &S, SimDev *dst, int *prefix, int nblocks, const double *dt, double dx,
+                               hipStream_t st) {
+    hipLaunchKernelGGL(k_direct_setup, dim3(1), dim3(64), 0, st, S, dst, prefix, nblocks, dt, dx);
+    return hipGetLastError();
+}
+
+hipError_t launch_direct_bits(const Ctrl *C, uint32_t *bits_out, hipStream_t st) {
+    hipLaunchKernelGGL(k_direct_bits, dim3(1), dim3(64), 0, st, C, bits_out);
     return hipGetLastError();
 }
 
