@@ -60,13 +60,15 @@ struct FarList {
 };
 
 // Workspace header of the single-grid entry points.
+// sim/prefix/ctrl: the single explicit-dt step through the whole-line
+// kernels; bits: signal-speed maximum of the CFL entry point; totals[0]: the
+// dt the step tuner times with; ctrl also serves the single-grid loop.
 struct WsHeader {
     SimDev sim;
     Ctrl ctrl;
     int prefix[2];
     uint32_t bits[4];
     double totals[5];
-    FarList far;
 };
 
 constexpr size_t kWsHeaderBytes = 1024;

@@ -1,12 +1,10 @@
 // gfx950 kernels of the jax_tinyfluids time step.
 //
-// The hot kernel is the fused split step (reference fluid.py:560-578): the X,
-// Y and Z sweeps of one time step in a single pass over HBM.
+// The hot kernels are the fused split step (reference fluid.py:560-578): the
+// X, Y and Z sweeps of one time step in a single pass over HBM.
 //
-//   Work unit: one wave64 owns a tile of TX x-columns by 62 z-columns and
-//   streams through a chunk of `ly` y-rows (plus one halo row on each side).
-//   Lanes run along z, the contiguous axis, so every load and store is a
-//   coalesced 256-byte row segment; lanes 0 and 63 are z-halo columns.
+//   Lanes run along z, the contiguous axis; a wave owns TX x-columns and
+//   streams along y through a chunk of rows (plus one halo row each side).
 //   - X sweep: each lane loads the TX+2 x-neighbours of its column for the
 //     current row and does the whole x-stencil in registers (x is the
 //     slowest axis, so its neighbours are other rows of the same lane).
@@ -14,24 +12,23 @@
 //     previous row of the same lane; its sweep quantities and the flux at the
 //     lower interface stay in registers between rows (Lie splitting: the Y
 //     sweep consumes the X-swept cells, fluid.py:574-575).
-//   - Z sweep: the z-neighbours are the adjacent lanes; sweep quantities move
-//     one lane with DPP wave_shr / wave_shl (fluid.py:576).
+//   - Z sweep: the z-neighbours are the adjacent lanes (fluid.py:576):
+//     through LDS when a workgroup holds whole z-lines (LineCtx, nz <= 1024),
+//     through DPP lane shifts when one wave holds a 62-cell z-tile with two
+//     halo lanes (HaloCtx).
 //   - Epilogue: each output cell contributes max_d |v_d| + c to the next CFL
 //     time step (fluid.py:425-504) through an integer max on float bits,
 //     which also propagates NaN, so the next dt needs no second read.
-//   Waves never exchange data, so the kernel has no barriers and no LDS.
 //
 // Algorithmic traffic is 40 bytes per cell-update (5 float32 read, 5 written);
-// the halo rows and columns are re-read from L2 / MALL.
+// the halo rows and columns are re-read from L2.
 #include "tf_internal.h"
 
-// Cache policy of the state stores. Plain write-back stores (0) let L2 merge
-// the partial 128-byte lines that two neighbouring z-tiles write (a tile's
-// 62 columns are not line-aligned); streaming (NT) stores write those halves
-// to HBM separately and measured ~25% slower.
-#ifndef TF_STORE_AUX
-#define TF_STORE_AUX 0
-#endif
+// Cache policy of the state stores: plain write-back stores, so that L2
+// merges the partial 128-byte lines that two neighbouring 62-cell z-tiles
+// write; streaming (NT) stores write those halves to HBM separately and
+// measured ~25% slower.
+constexpr int kStoreAux = 0;
 
 namespace tf {
 
@@ -44,21 +41,8 @@ __device__ __forceinline__ uint32_t wave_umax(uint32_t v) {
     return v;
 }
 
-// Global-address-space views of the state buffers: loads and stores then use
-// a scalar (SGPR) row base plus one 32-bit per-lane offset, with no per-lane
-// 64-bit address arithmetic.
-typedef __attribute__((address_space(1))) const float gcfloat;
-typedef __attribute__((address_space(1))) float gfloat;
-
-
-// Loads the 5 fields of one cell: `col` is the wave-uniform address of the
-// cell row's first field, `zb` the lane's byte offset along z (< 4 GiB), so
-// each load is a global_load with an SGPR base and a 32-bit VGPR offset.
-typedef __attribute__((address_space(1))) const char gcbyte;
-typedef __attribute__((address_space(1))) char gbyte;
-
-// A wave-uniform address pinned to SGPRs: the compiler cannot fold a lane
-// offset into it, so base + 32-bit lane offset selects the SGPR-base form.
+// A wave-uniform address pinned to SGPRs (readfirstlane of both halves), so
+// that everything derived from it stays scalar.
 template <typename T>
 __device__ __forceinline__ T *sgpr_ptr(T *p) {
     const uint64_t v = (uint64_t)p;
@@ -67,23 +51,13 @@ __device__ __forceinline__ T *sgpr_ptr(T *p) {
     return (T *)(((uint64_t)hi << 32) | lo);
 }
 
-#ifndef TF_LOAD_AUX_IN
-#define TF_LOAD_AUX_IN 0
-#endif
-#ifndef TF_LOAD_AUX_HALO
-#define TF_LOAD_AUX_HALO 0
-#endif
+// Raw buffer access: byte offset = voffset + soffset. Loads use the default
+// (cached) policy; streaming loads of any of the columns measured slower.
 __device__ __forceinline__ float buf_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
 }
-__device__ __forceinline__ float buf_load_in(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, TF_LOAD_AUX_IN));
-}
-__device__ __forceinline__ float buf_load_halo(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, TF_LOAD_AUX_HALO));
-}
 __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, float v) {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, (int)voff, (int)soff, TF_STORE_AUX);
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, (int)voff, (int)soff, kStoreAux);
 }
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t field_rsrc(const float *field) {
     return __builtin_amdgcn_make_buffer_rsrc((void *)field, (short)0, -1, 0x00020000);
@@ -144,8 +118,7 @@ struct HaloCtx {
             for (int f = 0; f < 5; ++f) w[f] = in_ptr[f * nc + ((size_t)soff + zb) / 4];
         } else {
 #pragma unroll
-            for (int f = 0; f < 5; ++f)
-                w[f] = (j >= 2 && j < TX) ? buf_load_in(rin[f], zb, soff) : buf_load_halo(rin[f], zb, soff);
+            for (int f = 0; f < 5; ++f) w[f] = buf_load(rin[f], zb, soff);
         }
     }
     // Z flux at the lower interface of this lane's cell (left state = lane - 1).
@@ -242,7 +215,7 @@ struct LineCtx {
     __device__ __forceinline__ void store(int s, int y, const float (&w)[5]) const {
         const uint32_t so = (uint32_t)y * row_b + (uint32_t)s * plane_b;
 #pragma unroll
-        for (int f = 0; f < 5; ++f) tf_struct_store_f32(w[f], rout[f], (int)xs_i, 0, (int)so, TF_STORE_AUX);
+        for (int f = 0; f < 5; ++f) tf_struct_store_f32(w[f], rout[f], (int)xs_i, 0, (int)so, kStoreAux);
     }
 };
 
@@ -438,23 +411,14 @@ __device__ __forceinline__ uint32_t fused_line(const float *in_, float *out_, in
     return stream_tile<TX>(T, g, yoff0);
 }
 
-#ifndef TF_TILE_ORDER
-#define TF_TILE_ORDER 0
-#endif
-// Tile order of consecutive waves: z-tiles fastest (0) or x-tiles fastest (1),
-// then the other, then y-chunks.
+// Tile order of consecutive waves: z-tiles fastest, then x-tiles, then
+// y-chunks. The four z-tiles of a workgroup share their boundary lines in
+// L1/L2; orders with x-tiles or y-chunks inside a workgroup measured slower.
 __device__ __forceinline__ void decode_tile(int local, int ntz, int ntx, int &xt, int &zt, int &yc) {
-#if TF_TILE_ORDER == 0
     zt = local % ntz;
     const int rest = local / ntz;
     xt = rest % ntx;
     yc = rest / ntx;
-#else
-    xt = local % ntx;
-    const int rest = local / ntx;
-    zt = rest % ntz;
-    yc = rest / ntz;
-#endif
 }
 
 // Wave index with an XCD-aware block order. Workgroups are dispatched to the
