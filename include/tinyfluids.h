@@ -211,7 +211,9 @@ int tf_autotune_evolve(const float *in, float *out, int nx, int ny, int nz, doub
 /* Times the candidate tilings of the step that tf_time_integration runs
  * (tile width 2/4/8 x y-chunk rows, halo or whole-z-line tiling) on this
  * state, keeps the fastest for this shape and reports it (tile_y as
- * resolved). Reads state, writes scratch; synchronous. No reference
+ * resolved). The halo/line choice also holds for tf_evolve_state (with a
+ * workspace) and for batch plans of this shape, until tf_set_tiling resets
+ * it to whole lines. Reads state, writes scratch; synchronous. No reference
  * counterpart (JAX compiles per shape on the first call instead). */
 int tf_autotune_loop(float *state, float *scratch, int nx, int ny, int nz, double grid_spacing, double gamma,
                      int handle_boundaries, void *workspace, int *tile_x, int *tile_y, float *ms_per_step,

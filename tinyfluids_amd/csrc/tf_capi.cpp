@@ -115,7 +115,13 @@ int default_tile_y(int nx, int ny, int nz, int tx) {
 
 // Tiling of the one-simulation plan that tf_time_integration runs (set by
 // tf_autotune_loop or tf_set_tiling); tile_y 0 lets the plan choose.
-std::map<std::tuple<int, int, int>, std::pair<int, int>> g_loop_tiling;
+struct LoopTiling {
+    int tx, ly;
+    int line;  // 1: whole z-lines where the shape allows them; 0: 62-cell z-tiles
+};
+std::map<std::tuple<int, int, int>, LoopTiling> g_loop_tiling;
+// Set by the loop tuner while it builds a candidate plan: -1 none, else 0/1.
+thread_local int tl_line_force = -1;
 
 // Untuned loop tiling: narrow tiles for grids too small to fill the chip,
 // so that each wave's serial stream stays short.
@@ -131,8 +137,8 @@ void loop_tiling_for(int nx, int ny, int nz, int &tx, int &ly) {
         std::lock_guard<std::mutex> lk(g_tiling_mu);
         auto it = g_loop_tiling.find({nx, ny, nz});
         if (it != g_loop_tiling.end()) {
-            tx = it->second.first;
-            ly = it->second.second;
+            tx = it->second.tx;
+            ly = it->second.ly;
             return;
         }
         auto jt = g_tiling.find({nx, ny, nz});
@@ -244,10 +250,20 @@ double line_lane_use(int nz) {
     return (double)(g * nz) / (64.0 * w);
 }
 
-// Whole z-lines whenever they fit a workgroup and use the lanes at least as
-// well as 62-cell z-tiles: every row segment then starts at its line's start
-// instead of one cell before a 62-cell tile.
-bool use_line_tiling(int nz) {
+// Whole z-lines where they fit a workgroup and use the lanes at least as well
+// as 62-cell z-tiles (every row segment then starts at its line's start
+// instead of one cell before a 62-cell tile), unless the loop tuner measured
+// 62-cell tiles faster for this shape (few waves per line leave too few
+// waves per CU; 384^3 is one such shape).
+bool use_line_tiling(int nx, int ny, int nz) {
+    if (g_line_mode.load() == 0) return false;
+    if (!(nz <= 64 * kMaxLineW && line_lane_use(nz) >= halo_lane_use(nz) - 1e-9)) return false;
+    if (tl_line_force >= 0) return tl_line_force == 1;
+    std::lock_guard<std::mutex> lk(g_tiling_mu);
+    auto it = g_loop_tiling.find({nx, ny, nz});
+    return it == g_loop_tiling.end() || it->second.line != 0;
+}
+bool line_allowed(int nz) {
     return g_line_mode.load() != 0 && nz <= 64 * kMaxLineW && line_lane_use(nz) >= halo_lane_use(nz) - 1e-9;
 }
 }  // namespace
@@ -328,7 +344,7 @@ int tf_evolve_state(const float *in, float *out, int nx, int ny, int nz, double 
     if (int e = check_open_bc(handle_boundaries, nx, ny, nz)) return e;
     int tx, ly;
     tiling_for(nx, ny, nz, tx, ly);
-    if (workspace && use_line_tiling(nz)) {
+    if (workspace && use_line_tiling(nx, ny, nz)) {
         // Whole z-lines, as in loops and batches: the table and the control
         // block (step 0, k from dt) are written on the stream by a setup
         // kernel, so the call stays asynchronous.
@@ -384,7 +400,7 @@ int tf_set_tiling(int nx, int ny, int nz, int tile_x, int tile_y) {
     TF_REQUIRE(tile_y >= 1, "tile_y must be >= 1");
     std::lock_guard<std::mutex> lk(g_tiling_mu);
     g_tiling[{nx, ny, nz}] = {tile_x, std::min(tile_y, ny)};
-    g_loop_tiling[{nx, ny, nz}] = {tile_x, std::min(tile_y, ny)};
+    g_loop_tiling[{nx, ny, nz}] = {tile_x, std::min(tile_y, ny), 1};
     return TF_OK;
 }
 
@@ -548,8 +564,8 @@ int tf_batch_create(tf_batch **out, const tf_sim_desc *sims, int nsims, void *wo
         for (int i = 0; i < nsims; ++i) {
             auto it = g_loop_tiling.find({sims[i].nx, sims[i].ny, sims[i].nz});
             if (it == g_loop_tiling.end()) continue;
-            tuned_ly[i] = it->second.second;
-            if (i == largest) tuned_tx = it->second.first;
+            tuned_ly[i] = it->second.ly;
+            if (i == largest) tuned_tx = it->second.tx;
         }
     }
     // Default tile width: 8 for work that saturates HBM (fewest halo
@@ -575,7 +591,7 @@ int tf_batch_create(tf_batch **out, const tf_sim_desc *sims, int nsims, void *wo
             S.buf[0] = d.buf0;
             S.buf[1] = d.buf1;
             S.ctrl = reinterpret_cast<Ctrl *>(d.ctrl);
-            if (use_line_tiling(d.nz)) {
+            if (use_line_tiling(d.nx, d.ny, d.nz)) {
                 int w, groups;
                 line_shape(d.nz, w, groups);
                 const int txl = line_tx(w, tx);
@@ -959,16 +975,21 @@ int tf_autotune_loop(float *state, float *scratch, int nx, int ny, int nz, doubl
     TF_HIP(hipEventCreate(&e0));
     TF_HIP(hipEventCreate(&e1));
     float best = 1e30f;
-    int btx = default_loop_tx(nx, ny, nz), bly = 0;
+    int btx = default_loop_tx(nx, ny, nz), bly = 0, bline = 1;
     std::vector<std::tuple<int, int, int>> seen;  // (tx, resolved ly, line waves)
     int rc = TF_OK;
+    const std::vector<int> lines = line_allowed(nz) ? std::vector<int>{1, 0} : std::vector<int>{0};
+    for (size_t li = 0; li < lines.size() && rc == TF_OK; ++li)
     for (int tx : txs) {
+        const int line = lines[li];
         if (tx > 2 && tx > nx) continue;
         for (int ly : lys) {
             if (ly > ny) continue;
             PlanGuard plan;
+            tl_line_force = line;
             rc = tf_batch_create(&plan.b, &d, 1, plan_workspace(workspace), tf_batch_workspace_bytes(1), tx, ly,
                                  stream);
+            tl_line_force = -1;
             if (rc != TF_OK) break;
             const SimDev &S = plan.b->h_sims[0];
             const auto key = std::make_tuple(S.line_w > 0 ? line_tx(S.line_w, tx) : tx, S.ly, S.line_w);
@@ -991,6 +1012,7 @@ int tf_autotune_loop(float *state, float *scratch, int nx, int ny, int nz, doubl
                 best = ms;
                 btx = tx;
                 bly = S.ly;
+                bline = S.line_w > 0 ? 1 : 0;
             }
         }
         if (rc != TF_OK) break;
@@ -1000,7 +1022,7 @@ int tf_autotune_loop(float *state, float *scratch, int nx, int ny, int nz, doubl
     if (rc != TF_OK) return rc;
     {
         std::lock_guard<std::mutex> lk(g_tiling_mu);
-        g_loop_tiling[{nx, ny, nz}] = {btx, bly};
+        g_loop_tiling[{nx, ny, nz}] = {btx, bly, bline};
     }
     *tile_x = btx;
     *tile_y = bly;
