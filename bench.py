@@ -232,10 +232,13 @@ def plan_kernel(tx, waves_per_line):
     return "tf::k_line_wide<4, 8>" if tx == 4 else "tf::k_line_wide<2, 16>"
 
 
-def measure_grid(dev, nx, ny, nz, steps, warmup, tile, no_tune, calibrate):
+def measure_grid(dev, nx, ny, nz, steps, warmup, tile, no_tune, calibrate, copy_ceiling=False):
     """KH on nx x ny x nz resident in HBM: tiling (forced, measured by the
     loop tuner, or the default), `warmup` steps, then `steps` timed steps
-    (wall clock and HIP events around the stencil launches)."""
+    (wall clock and HIP events around the stencil launches). With
+    copy_ceiling, also the rate of a flat copy of the state buffer
+    (tf_hbm_copy: every byte read once and written once), the practical
+    ceiling of a read+write kernel on this device."""
     import ctypes
 
     import torch
@@ -263,6 +266,19 @@ def measure_grid(dev, nx, ny, nz, steps, warmup, tile, no_tune, calibrate):
         buf1.copy_(buf0)
         torch.cuda.synchronize()
         print(f"bench: calibration copy of {buf0.numel() * 4} bytes", file=sys.stderr, flush=True)
+    copy_gbs = None
+    if copy_ceiling:
+        nbytes = buf0.numel() * 4
+        stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        _lib.check(lib.tf_hbm_copy(buf0.data_ptr(), buf1.data_ptr(), nbytes, stream))
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(3):
+            _lib.check(lib.tf_hbm_copy(buf0.data_ptr(), buf1.data_ptr(), nbytes, stream))
+        e1.record()
+        torch.cuda.synchronize()
+        copy_gbs = 2 * nbytes / (e0.elapsed_time(e1) / 3 / 1e3) / 1e9
+        print(f"bench: flat copy of {nbytes} bytes: {copy_gbs:.0f} GB/s (read + write)", file=sys.stderr, flush=True)
     sim = Simulation(buf0, dx, gamma, buffers=(buf0, buf1))
     batch = Batch([sim], tile_x=tx, tile_y=ty)
     ptx, pty, pw = batch.tiling(0)
@@ -281,7 +297,7 @@ def measure_grid(dev, nx, ny, nz, steps, warmup, tile, no_tune, calibrate):
     del buf0, buf1, sim, batch
     torch.cuda.empty_cache()
     return {"tile_x": ptx, "tile_y": pty, "waves_per_line": pw, "kernel": plan_kernel(ptx, pw), "status": st,
-            "seconds": el, "stencil_ms": stencil_ms}
+            "seconds": el, "stencil_ms": stencil_ms, "copy_gbs": copy_gbs}
 
 
 def main():
@@ -294,7 +310,8 @@ def main():
 
     nx, ny, nz = (int(v) for v in args.shape.split(","))
     dev = torch.device("cuda:0")
-    m = measure_grid(dev, nx, ny, nz, args.steps, args.warmup, args.tile, args.no_tune, args.calibrate_copy)
+    m = measure_grid(dev, nx, ny, nz, args.steps, args.warmup, args.tile, args.no_tune, args.calibrate_copy,
+                     copy_ceiling=True)
     tx, ly, st, el, stencil_ms = m["tile_x"], m["tile_y"], m["status"], m["seconds"], m["stencil_ms"]
     kernel = m["kernel"]
     cells = nx * ny * nz
@@ -322,7 +339,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": kernel, "kernel_ms": kernel_s * 1e3,
-                     "algorithmic_bytes_per_launch": BYTES_PER_CELL * cells},
+                     "algorithmic_bytes_per_launch": BYTES_PER_CELL * cells,
+                     "copy_ceiling": m["copy_gbs"],
+                     "frac_of_copy_ceiling": achieved / m["copy_gbs"] if m["copy_gbs"] else None},
         "sim_time": st.t,
     }
     if not args.no_shapes and (nx, ny, nz) == (1024, 1024, 1024):

@@ -198,6 +198,13 @@ int tf_batch_status(tf_batch *b, tf_sim_status *host_out, void *stream);
  * Tiling and autotuning of the fused stencil. No reference counterpart.
  * ------------------------------------------------------------------------ */
 
+/* Copies bytes (a multiple of 16, 16-byte aligned, non-overlapping) with the
+ * flat one-float4-per-thread kernel: one read and one write of every byte.
+ * Its rate is the practical HBM ceiling of a read+write kernel that the
+ * fused step's rate is compared with. Asynchronous on the stream. No
+ * reference counterpart. */
+int tf_hbm_copy(const void *src, void *dst, size_t bytes, void *stream);
+
 /* Tiling used for an nx*ny*nz grid by tf_evolve_state; tf_set_tiling also
  * fixes it for tf_time_integration. */
 int tf_get_tiling(int nx, int ny, int nz, int *tile_x, int *tile_y);

@@ -49,6 +49,10 @@ def test_argument_errors_without_gpu():
     assert lib.tf_set_tiling(8, 8, 8, 3, 8) == _lib.TF_ERR_ARG
     assert lib.tf_workspace_bytes(8, 8, 8) >= 1024
     assert lib.tf_batch_workspace_bytes(4) > 0
+    assert lib.tf_hbm_copy(ctypes.c_void_p(16), ctypes.c_void_p(4096), 24, None) == _lib.TF_ERR_ARG
+    assert b"16" in lib.tf_last_error()
+    assert lib.tf_hbm_copy(ctypes.c_void_p(16), ctypes.c_void_p(32), 64, None) == _lib.TF_ERR_ARG
+    assert b"overlap" in lib.tf_last_error()
     with pytest.raises(ValueError):
         _lib.check(_lib.TF_ERR_ARG)
 

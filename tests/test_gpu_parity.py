@@ -52,6 +52,19 @@ def test_library_is_native():
     assert torch.cuda.is_available()
 
 
+@pytest.mark.parametrize("nfloats", [4, 1 << 20, 5 * 33 * 17 * 4 + 4])
+def test_hbm_copy_copies_every_byte(nfloats):
+    import ctypes
+    import tinyfluids_amd._lib as L
+    src = torch.randn(nfloats, device=DEV)
+    dst = torch.full((nfloats + 4,), -7.0, device=DEV)
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    L.check(L.load().tf_hbm_copy(src.data_ptr(), dst.data_ptr(), nfloats * 4, stream))
+    torch.cuda.synchronize()
+    assert torch.equal(dst[:nfloats], src)
+    assert torch.all(dst[nfloats:] == -7.0)
+
+
 @pytest.mark.parametrize("gamma", [5 / 3, 1.4])
 def test_conversions(gamma):
     s = random_state((6, 7, 9), 1)

@@ -76,6 +76,7 @@ SIGNATURES = {
     "tf_batch_status": (I, [P, ctypes.POINTER(SimStatus), P]),
     "tf_batch_step_timed": (I, [P, I, ctypes.POINTER(ctypes.c_float), P]),
     "tf_set_line_mode": (I, [I]),
+    "tf_hbm_copy": (I, [P, P, SZ, P]),
     "tf_get_tiling": (I, [I, I, I, ctypes.POINTER(I), ctypes.POINTER(I)]),
     "tf_set_tiling": (I, [I, I, I, I, I]),
     "tf_autotune_evolve": (I, [P, P, I, I, I, D, D, P, ctypes.POINTER(I), ctypes.POINTER(I),

@@ -35,6 +35,7 @@ hipError_t launch_direct_bits(const Ctrl *C, uint32_t *bits_out, hipStream_t st)
 hipError_t launch_finalize_one(Ctrl *C, int first, hipStream_t st);
 hipError_t launch_ctrl_reset(Ctrl *C, double dx, double c_cfl, double t_final, int64_t max_steps, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, int n, hipStream_t st);
+hipError_t launch_copy16(const void *src, void *dst, int64_t n16, hipStream_t st);
 hipError_t launch_speed(const float *prim, int64_t nc, float gamma, uint32_t *bits, hipStream_t st);
 hipError_t launch_dt_from_bits(const uint32_t *bits, double dx, double c_cfl, double *dt_out, float *max_out,
                                hipStream_t st);
@@ -387,6 +388,18 @@ int tf_conserved_totals(const float *prim, int nx, int ny, int nz, double gamma,
 }
 
 // ------------------------------------------------------------------ tiling
+int tf_hbm_copy(const void *src, void *dst, size_t bytes, void *stream) {
+    TF_REQUIRE(src && dst, "null pointer");
+    TF_REQUIRE(bytes % 16 == 0 && reinterpret_cast<uintptr_t>(src) % 16 == 0 &&
+                   reinterpret_cast<uintptr_t>(dst) % 16 == 0,
+               "tf_hbm_copy needs 16-byte aligned pointers and a size in multiples of 16 bytes");
+    const uintptr_t s0 = reinterpret_cast<uintptr_t>(src), d0 = reinterpret_cast<uintptr_t>(dst);
+    TF_REQUIRE(s0 + bytes <= d0 || d0 + bytes <= s0, "tf_hbm_copy buffers overlap");
+    if (bytes == 0) return TF_OK;
+    TF_HIP(launch_copy16(src, dst, (int64_t)(bytes / 16), S_(stream)));
+    return TF_OK;
+}
+
 int tf_get_tiling(int nx, int ny, int nz, int *tile_x, int *tile_y) {
     if (int e = check_dims(nx, ny, nz)) return e;
     TF_REQUIRE(tile_x && tile_y, "null pointer");
