@@ -170,6 +170,28 @@ def measured_traffic(nx, ny, nz, kernel, tile):
     return int(fields["traffic_bytes_per_launch"]), os.path.relpath(path, ROOT)
 
 
+def measured_valu(nx, ny, nz, kernel, tile):
+    """VALU instructions per cell-update and the VALU-busy fraction of the
+    fused step from the committed rocprofv3 SQ-counter summary, when it was
+    taken on this workload, kernel and tiling; else None."""
+    path = os.path.join(ROOT, "profiles", "fused_step_kh1024_valu.md")
+    if (nx, ny, nz) != (1024, 1024, 1024) or not os.path.exists(path):
+        return None
+    text = open(path).read()
+    if f"`{kernel}`" not in text or f"(tile_x,tile_y = {tile[0]},{tile[1]})" not in text:
+        return None
+    out = {"source": os.path.relpath(path, ROOT)}
+    for line in text.splitlines():
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) != 2:
+            continue
+        if cells[0].startswith("VALU instructions per cell-update"):
+            out["insts_per_cell"] = float(cells[1])
+        elif cells[0].startswith("SQ_ACTIVE_INST_VALU"):
+            out["busy"] = float(cells[1])
+    return out if len(out) == 3 else None
+
+
 def batched_leg(dev):
     """256 uneven blasts, sides {32,48,64,96,128}^3, 100 steps each: all
     advanced together by one captured HIP graph, against the plain path that
@@ -341,6 +363,7 @@ def main():
                      "kernel": kernel, "kernel_ms": kernel_s * 1e3,
                      "algorithmic_bytes_per_launch": BYTES_PER_CELL * cells,
                      "copy_ceiling": m["copy_gbs"],
+                     "valu": measured_valu(nx, ny, nz, kernel, (tx, ly)),
                      "frac_of_copy_ceiling": achieved / m["copy_gbs"] if m["copy_gbs"] else None},
         "sim_time": st.t,
     }
