@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cstdarg>
+#include <cstddef>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -251,21 +252,21 @@ double line_lane_use(int nz) {
     return (double)(g * nz) / (64.0 * w);
 }
 
-// Whole z-lines where they fit a workgroup and use the lanes at least as well
-// as 62-cell z-tiles (every row segment then starts at its line's start
-// instead of one cell before a 62-cell tile), unless the loop tuner measured
-// 62-cell tiles faster for this shape (few waves per line leave too few
-// waves per CU; 384^3 is one such shape).
+// Whole z-lines are possible where they fit a workgroup and use the lanes at
+// least as well as 62-cell z-tiles (every row segment then starts at its
+// line's start instead of one cell before a 62-cell tile).
+bool line_allowed(int nz) {
+    return g_line_mode.load() != 0 && nz <= 64 * kMaxLineW && line_lane_use(nz) >= halo_lane_use(nz) - 1e-9;
+}
+// ... and used unless the loop tuner measured 62-cell tiles faster for this
+// shape (few waves per line leave too few waves per CU; 384^3 is one such
+// shape).
 bool use_line_tiling(int nx, int ny, int nz) {
-    if (g_line_mode.load() == 0) return false;
-    if (!(nz <= 64 * kMaxLineW && line_lane_use(nz) >= halo_lane_use(nz) - 1e-9)) return false;
+    if (!line_allowed(nz)) return false;
     if (tl_line_force >= 0) return tl_line_force == 1;
     std::lock_guard<std::mutex> lk(g_tiling_mu);
     auto it = g_loop_tiling.find({nx, ny, nz});
     return it == g_loop_tiling.end() || it->second.line != 0;
-}
-bool line_allowed(int nz) {
-    return g_line_mode.load() != 0 && nz <= 64 * kMaxLineW && line_lane_use(nz) >= halo_lane_use(nz) - 1e-9;
 }
 }  // namespace
 
@@ -387,7 +388,7 @@ int tf_conserved_totals(const float *prim, int nx, int ny, int nz, double gamma,
     return TF_OK;
 }
 
-// ------------------------------------------------------------------ tiling
+// ------------------------------------------------------------------ HBM copy
 int tf_hbm_copy(const void *src, void *dst, size_t bytes, void *stream) {
     TF_REQUIRE(src && dst, "null pointer");
     TF_REQUIRE(bytes % 16 == 0 && reinterpret_cast<uintptr_t>(src) % 16 == 0 &&
@@ -400,6 +401,7 @@ int tf_hbm_copy(const void *src, void *dst, size_t bytes, void *stream) {
     return TF_OK;
 }
 
+// ------------------------------------------------------------------ tiling
 int tf_get_tiling(int nx, int ny, int nz, int *tile_x, int *tile_y) {
     if (int e = check_dims(nx, ny, nz)) return e;
     TF_REQUIRE(tile_x && tile_y, "null pointer");
@@ -744,10 +746,8 @@ int tf_batch_destroy(tf_batch *b) {
     return TF_OK;
 }
 
-// One batched stencil step: the halo-tiled class, each line-tiled class, then
-// the far-row tiles (all before the dt update).
-// The k-th non-empty stencil launch of a step (tiling classes in order, then
-// the far rows) on stream st.
+// The k-th non-empty stencil launch of a step (the halo-tiled class, each
+// line-tiled class in order, then the far rows) on stream st.
 static hipError_t launch_stencil_part(tf_batch *b, int k, hipStream_t st) {
     int i = 0;
     for (int c = 0; c < kClasses; ++c) {
@@ -875,6 +875,15 @@ int tf_batch_step_timed(tf_batch *b, int steps, float *stencil_ms, void *stream)
     return rc;
 }
 
+// tf_batch_status copies the first sizeof(tf_sim_status) bytes of each Ctrl.
+static_assert(sizeof(tf_sim_status) == 48 && offsetof(Ctrl, t) == offsetof(tf_sim_status, t) &&
+                  offsetof(Ctrl, dt) == offsetof(tf_sim_status, dt) &&
+                  offsetof(Ctrl, n_iter) == offsetof(tf_sim_status, n_iter) &&
+                  offsetof(Ctrl, active) == offsetof(tf_sim_status, active) &&
+                  offsetof(Ctrl, nonfinite) == offsetof(tf_sim_status, nonfinite) &&
+                  offsetof(Ctrl, max_speed) == offsetof(tf_sim_status, max_speed),
+              "Ctrl must begin with the layout of tf_sim_status");
+
 int tf_batch_status(tf_batch *b, tf_sim_status *host_out, void *stream) {
     TF_REQUIRE(b && host_out, "null pointer");
     hipStream_t st = S_(stream);
@@ -992,41 +1001,42 @@ int tf_autotune_loop(float *state, float *scratch, int nx, int ny, int nz, doubl
     std::vector<std::tuple<int, int, int>> seen;  // (tx, resolved ly, line waves)
     int rc = TF_OK;
     const std::vector<int> lines = line_allowed(nz) ? std::vector<int>{1, 0} : std::vector<int>{0};
-    for (size_t li = 0; li < lines.size() && rc == TF_OK; ++li)
-    for (int tx : txs) {
-        const int line = lines[li];
-        if (tx > 2 && tx > nx) continue;
-        for (int ly : lys) {
-            if (ly > ny) continue;
-            PlanGuard plan;
-            tl_line_force = line;
-            rc = tf_batch_create(&plan.b, &d, 1, plan_workspace(workspace), tf_batch_workspace_bytes(1), tx, ly,
-                                 stream);
-            tl_line_force = -1;
+    for (int line : lines) {
+        for (int tx : txs) {
+            if (tx > 2 && tx > nx) continue;
+            for (int ly : lys) {
+                if (ly > ny) continue;
+                PlanGuard plan;
+                tl_line_force = line;
+                rc = tf_batch_create(&plan.b, &d, 1, plan_workspace(workspace), tf_batch_workspace_bytes(1), tx, ly,
+                                     stream);
+                tl_line_force = -1;
+                if (rc != TF_OK) break;
+                const SimDev &S = plan.b->h_sims[0];
+                const auto key = std::make_tuple(S.line_w > 0 ? line_tx(S.line_w, tx) : tx, S.ly, S.line_w);
+                if (std::find(seen.begin(), seen.end(), key) != seen.end()) continue;
+                seen.push_back(key);
+                const int reps = big ? 2 : 5;
+                hipError_t e = launch_stencils(plan.b, st, false);
+                if (e == hipSuccess) e = hipEventRecord(e0, st);
+                for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_stencils(plan.b, st, false);
+                if (e == hipSuccess) e = hipEventRecord(e1, st);
+                if (e == hipSuccess) e = hipEventSynchronize(e1);
+                float ms = 0.f;
+                if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+                if (e != hipSuccess) {
+                    rc = fail(TF_ERR_HIP, "autotune launch: %s", hipGetErrorString(e));
+                    break;
+                }
+                ms /= reps;
+                if (ms < best) {
+                    best = ms;
+                    btx = tx;
+                    bly = S.ly;
+                    bline = S.line_w > 0 ? 1 : 0;
+                }
+            }
             if (rc != TF_OK) break;
-            const SimDev &S = plan.b->h_sims[0];
-            const auto key = std::make_tuple(S.line_w > 0 ? line_tx(S.line_w, tx) : tx, S.ly, S.line_w);
-            if (std::find(seen.begin(), seen.end(), key) != seen.end()) continue;
-            seen.push_back(key);
-            const int reps = big ? 2 : 5;
-            hipError_t e = launch_stencils(plan.b, st, false);
-            if (e == hipSuccess) e = hipEventRecord(e0, st);
-            for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_stencils(plan.b, st, false);
-            if (e == hipSuccess) e = hipEventRecord(e1, st);
-            if (e == hipSuccess) e = hipEventSynchronize(e1);
-            float ms = 0.f;
-            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-            if (e != hipSuccess) {
-                rc = fail(TF_ERR_HIP, "autotune launch: %s", hipGetErrorString(e));
-                break;
-            }
-            ms /= reps;
-            if (ms < best) {
-                best = ms;
-                btx = tx;
-                bly = S.ly;
-                bline = S.line_w > 0 ? 1 : 0;
-            }
         }
         if (rc != TF_OK) break;
     }
