@@ -176,8 +176,11 @@ int tf_batch_sim_tiling(const tf_batch *b, int sim, int *tile_x, int *tile_y, in
  * Simulations that are no longer active are skipped on the device. */
 int tf_batch_step(tf_batch *b, int steps, void *stream);
 
-/* As tf_batch_step, but the `steps` steps are captured once into a HIP graph
- * (re-captured when `steps` changes) and replayed: one graph launch. */
+/* As tf_batch_step, but the steps run as replays of captured HIP graphs:
+ * one of 16 steps (fewer when `steps` < 16) launched steps / 16 times, and
+ * one of the remainder. Each graph is captured on first use and again when
+ * its step count changes; the stencil launches of a step run concurrently
+ * inside it. */
 int tf_batch_step_graph(tf_batch *b, int steps, void *stream);
 
 /* As tf_batch_step, also recording HIP events around every stencil launch
@@ -185,11 +188,17 @@ int tf_batch_step_graph(tf_batch *b, int steps, void *stream);
  * duration of the stencil launches alone (without the dt updates). */
 int tf_batch_step_timed(tf_batch *b, int steps, float *stencil_ms, void *stream);
 
-/* Process-wide default for plans built afterwards: 1 (default) lets a plan
- * run small grids (nz <= 256) with the line tiling, whole periodic z-lines
- * per workgroup, where that keeps more lanes busy than 62-column tiles;
- * 0 uses the 62-column tiling for every simulation. Results are identical. */
+/* Process-wide default for plans and single steps built afterwards: 1
+ * (default) runs grids with nz <= 1024 in the line tiling, whole periodic
+ * z-lines per workgroup, where that keeps at least as many lanes busy as
+ * 62-column tiles (and the loop tuner did not measure 62-column tiles faster
+ * for the shape); 0 uses the 62-column tiling everywhere. Results are
+ * identical. */
 int tf_set_line_mode(int enabled);
+
+/* tf_sim_reset of every simulation of the plan, in three launches (control
+ * blocks, signal-speed maxima of every buf0, first dt). Asynchronous. */
+int tf_batch_reset(tf_batch *b, void *stream);
 
 /* Copies every simulation's status to host memory (synchronous). */
 int tf_batch_status(tf_batch *b, tf_sim_status *host_out, void *stream);

@@ -50,8 +50,10 @@ def test_batched_tilings_bitwise(tile_x, tile_y):
 
 
 def test_graph_replays_match_eager():
-    """The captured graph is replayed several times, with eager work and
-    synchronisation in between; every replay matches eager stepping."""
+    """The captured graphs are replayed several times, with eager work and
+    synchronisation in between; every replay matches eager stepping. Step
+    counts cover one short graph, 16-step chunks plus a remainder graph, and
+    re-capture when the counts change."""
     from tinyfluids_amd.batch import Batch, Simulation
     specs = specs_mixed()[:3]
     a = [Simulation(s.initial_state(), s.dx, s.gamma, device=DEV) for s in specs]
@@ -59,16 +61,46 @@ def test_graph_replays_match_eager():
     ba, bb = Batch(a), Batch(b)
     for s in a + b:
         s.reset()
-    for rep in range(3):
-        ba.step_graph(4)
-        bb.step(4)
+    done = 0
+    for rep, steps in enumerate((4, 4, 37, 16, 5, 37)):
+        ba.step_graph(steps)
+        bb.step(steps)
+        done += steps
         torch.cuda.synchronize()
         _ = torch.ones(1000, device=DEV).sum().item()  # eager work between replays
         sa, sb = ba.status(), bb.status()
         for i in range(len(specs)):
-            assert sa[i].n_iter == sb[i].n_iter == 4 * (rep + 1)
+            assert sa[i].n_iter == sb[i].n_iter == done
             assert sa[i].t == sb[i].t
             assert torch.equal(a[i].state(sa[i]), b[i].state(sb[i])), (rep, i)
+
+
+def test_batch_reset_equals_per_simulation_reset():
+    """tf_batch_reset leaves every control block as tf_sim_reset does (t, dt,
+    step count, flag, signal speed), also after steps were taken."""
+    from tinyfluids_amd.batch import Batch, Simulation
+    specs = specs_mixed()
+    a = [Simulation(s.initial_state(), s.dx, s.gamma, max_steps=s.steps, device=DEV) for s in specs]
+    b = [Simulation(s.initial_state(), s.dx, s.gamma, max_steps=s.steps, device=DEV) for s in specs]
+    ba, bb = Batch(a), Batch(b)
+    for s in a:
+        s.reset()
+    ba.step(3)
+    for s in a:
+        s.reset()
+    bb.reset()
+    bb.step(3)
+    bb.reset()
+    sa, sb = ba.status(), bb.status()
+    for i in range(len(specs)):
+        for f in ("t", "dt", "n_iter", "active", "nonfinite", "max_speed"):
+            assert getattr(sa[i], f) == getattr(sb[i], f), (i, f)
+        assert sa[i].n_iter == 0 and sa[i].active and sa[i].dt > 0
+    ba.step(5)
+    bb.step(5)
+    sa, sb = ba.status(), bb.status()
+    for i in range(len(specs)):
+        assert torch.equal(a[i].state(sa[i]), b[i].state(sb[i])), i
 
 
 def test_batched_meets_oracle():
@@ -114,6 +146,9 @@ def test_server_respects_hbm_budget():
 LINE_SHAPES = [((5, 6, 8), False), ((7, 5, 16), True), ((6, 4, 24), False), ((4, 9, 32), False),
                ((3, 5, 48), True), ((6, 3, 64), False), ((3, 4, 100), False), ((4, 3, 128), True),
                ((3, 3, 192), False), ((2, 3, 256), False), ((9, 2, 20), False),
+               # several whole lines per workgroup across waves
+               ((9, 4, 48), False), ((13, 3, 96), True), ((11, 3, 80), False), ((7, 3, 33), True),
+               ((17, 2, 5), False),
                # 5..16 waves per z-line
                ((3, 4, 300), False), ((4, 3, 512), True), ((2, 5, 520), False), ((2, 4, 700), True),
                ((3, 3, 1024), False), ((2, 3, 1000), True)]

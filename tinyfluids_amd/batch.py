@@ -4,9 +4,10 @@ tf_batch_*).
 A ``Simulation`` owns two state buffers and a 128-byte control block in HBM;
 its t, dt, step count and loop flag live on the device, so stepping never
 waits for the host. A ``Batch`` advances any set of simulations, of any
-sizes, together: one fused stencil launch covers the tiles of every
-simulation, then one launch updates every simulation's dt. ``step_graph``
-replays a captured HIP graph of many such steps as one launch.
+sizes, together: one fused stencil launch per tiling class covers the tiles
+of every simulation, then one launch updates every simulation's dt.
+``step_graph`` replays captured HIP graphs of 16 such steps, and ``reset``
+resets every simulation in three launches.
 
 Each cell's arithmetic is the same whatever else is in the batch, so a
 simulation's result is bit-identical alone and in any mix.
@@ -108,6 +109,10 @@ class Batch:
         _lib.check(_lib.load().tf_batch_sim_tiling(self._h, int(index), ctypes.byref(tx), ctypes.byref(ty),
                                                    ctypes.byref(w)))
         return tx.value, ty.value, w.value
+
+    def reset(self):
+        """Simulation.reset of every simulation, in three launches (asynchronous)."""
+        _lib.check(_lib.load().tf_batch_reset(self._h, _stream(self.device)))
 
     def step(self, steps: int = 1):
         _lib.check(_lib.load().tf_batch_step(self._h, int(steps), _stream(self.device)))

@@ -30,6 +30,8 @@ hipError_t launch_step_sims_far(int tx, const SimDev *sims, const FarTile *far, 
 hipError_t launch_line_sims(int tx, int w, const SimDev *sims, const int *prefix, int nsims, int nblocks,
                             hipStream_t st);
 hipError_t launch_finalize_sims(const SimDev *sims, int nsims, hipStream_t st);
+hipError_t launch_reset_sims(const SimDev *sims, const ResetParams *rp, const int *bprefix, int nsims,
+                             int speed_blocks, hipStream_t st);
 hipError_t launch_direct_setup(const SimDev &S, SimDev *dst, int *prefix, int nblocks, const double *dt, double dx,
                                hipStream_t st);
 hipError_t launch_direct_bits(const Ctrl *C, uint32_t *bits_out, hipStream_t st);
@@ -235,11 +237,25 @@ constexpr int kMaxLineW = 16;          // whole z-lines of up to 1024 cells per 
 constexpr int kClasses = kMaxLineW + 1;  // halo tiling + line tiling with W = 1..16
 std::atomic<int> g_line_mode{1};
 
-// Line tiling of a z-line of nz cells: w waves per workgroup, and with
-// w == 1 as many x-column groups as whole lines fit in a wave.
+// Line tiling of a z-line of nz cells: w waves per workgroup holding
+// `groups` whole lines (x-column groups), lane after lane. The fewest waves
+// that hold a line, unless a few more (up to 8) hold several lines with at
+// least 2% fewer idle lanes: 48-cell lines go four to three waves, 96-cell
+// lines two to three waves (measured 6% faster on 48^3 and 96^3).
 void line_shape(int nz, int &w, int &groups) {
-    w = (nz + 63) / 64;
-    groups = w == 1 ? std::max(1, 64 / nz) : 1;
+    const int w0 = (nz + 63) / 64;
+    w = w0;
+    groups = std::max(1, 64 * w0 / nz);
+    double best = (double)(groups * nz) / (64.0 * w0);
+    for (int cw = w0 + 1; cw <= std::min(kMaxLineW, std::max(w0, 8)); ++cw) {
+        const int g = 64 * cw / nz;
+        const double use = (double)(g * nz) / (64.0 * cw);
+        if (use > best + 0.02) {
+            best = use;
+            w = cw;
+            groups = g;
+        }
+    }
 }
 double halo_lane_use(int nz) { return (double)nz / (64.0 * ((nz + kZT - 1) / kZT)); }
 // Tile width of a line class: at most 4 (registers for two waves per SIMD),
@@ -508,11 +524,17 @@ struct tf_batch {
     std::vector<int> order;      // h_sims[k] is the caller's simulation order[k]
     FarTile *d_far = nullptr;    // far-row tiles (table indices), see far_tiles
     int nfar = 0;
+    ResetParams *d_reset = nullptr;  // loop bounds per simulation (table order), for tf_batch_reset
+    int *d_speed_prefix = nullptr;   // signal-speed workgroups per simulation (prefix, table order)
+    int speed_blocks = 0;
     int far_max_ntz = 0;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    // Captured graphs of kGraphChunk steps (slot 0) and of a shorter
+    // remainder (slot 1), replayed by tf_batch_step_graph.
+    static constexpr int kGraphChunk = 16;
+    hipGraph_t graph[2] = {};
+    hipGraphExec_t exec[2] = {};
+    int graph_steps[2] = {};
     hipStream_t capture_stream = nullptr;
-    int graph_steps = 0;
     // Concurrent launches of the tiling classes of one step: side streams
     // (default priority) forked from and joined back into the caller's stream
     // with events; created on first use when a plan has several classes.
@@ -529,7 +551,8 @@ size_t tf_batch_workspace_bytes(int nsims) {
     if (nsims < 0) nsims = 0;
     return align256((size_t)nsims * sizeof(SimDev)) + align256((size_t)(nsims + kClasses) * sizeof(int)) +
            align256((size_t)nsims * sizeof(Ctrl)) +
-           align256((size_t)nsims * kMaxFarPerSim * kFarSplit * sizeof(FarTile));
+           align256((size_t)nsims * kMaxFarPerSim * kFarSplit * sizeof(FarTile)) +
+           align256((size_t)nsims * sizeof(ResetParams)) + align256((size_t)(nsims + 1) * sizeof(int));
 }
 
 
@@ -693,6 +716,24 @@ int tf_batch_create(tf_batch **out, const tf_sim_desc *sims, int nsims, void *wo
     b->d_status = reinterpret_cast<Ctrl *>(p);
     p += align256((size_t)nsims * sizeof(Ctrl));
     b->d_far = reinterpret_cast<FarTile *>(p);
+    p += align256((size_t)nsims * kMaxFarPerSim * kFarSplit * sizeof(FarTile));
+    b->d_reset = reinterpret_cast<ResetParams *>(p);
+    p += align256((size_t)nsims * sizeof(ResetParams));
+    b->d_speed_prefix = reinterpret_cast<int *>(p);
+    std::vector<ResetParams> rp(nsims);
+    std::vector<int> sp(nsims + 1, 0);
+    int64_t speed_blocks = 0;
+    for (int k = 0; k < nsims; ++k) {
+        const tf_sim_desc &d = sims[b->order[k]];
+        rp[k] = ResetParams{d.grid_spacing, d.c_cfl, d.t_final, d.max_steps};
+        speed_blocks += ((int64_t)d.nx * d.ny * d.nz + kSpeedCellsPerBlock - 1) / kSpeedCellsPerBlock;
+        if (speed_blocks > 2000000000) {
+            delete b;
+            return fail(TF_ERR_ARG, "batch too large");
+        }
+        sp[k + 1] = (int)speed_blocks;
+    }
+    b->speed_blocks = (int)speed_blocks;
     std::vector<FarTile> far;
     for (int k = 0; k < nsims; ++k) {
         const size_t before = far.size();
@@ -709,6 +750,10 @@ int tf_batch_create(tf_batch **out, const tf_sim_desc *sims, int nsims, void *wo
                            st);
     if (e == hipSuccess && b->nfar > 0)
         e = hipMemcpyAsync(b->d_far, far.data(), far.size() * sizeof(FarTile), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->d_reset, rp.data(), rp.size() * sizeof(ResetParams), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->d_speed_prefix, sp.data(), sp.size() * sizeof(int), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
         delete b;
@@ -734,8 +779,10 @@ int tf_batch_sim_tiling(const tf_batch *b, int sim, int *tile_x, int *tile_y, in
 
 int tf_batch_destroy(tf_batch *b) {
     if (!b) return TF_OK;
-    if (b->exec) (void)hipGraphExecDestroy(b->exec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (int i = 0; i < 2; ++i) {
+        if (b->exec[i]) (void)hipGraphExecDestroy(b->exec[i]);
+        if (b->graph[i]) (void)hipGraphDestroy(b->graph[i]);
+    }
     if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
     for (int i = 0; i < tf_batch::kSide; ++i) {
         if (b->side[i]) (void)hipStreamDestroy(b->side[i]);
@@ -810,33 +857,47 @@ int tf_batch_step(tf_batch *b, int steps, void *stream) {
     return enqueue_steps(b, steps, S_(stream));
 }
 
+// Graph slot `slot` holding `steps` captured steps (captured on first use
+// and when the step count changes).
+static int ensure_graph(tf_batch *b, int slot, int steps) {
+    if (b->exec[slot] && b->graph_steps[slot] == steps) return TF_OK;
+    if (b->exec[slot]) {
+        (void)hipGraphExecDestroy(b->exec[slot]);
+        b->exec[slot] = nullptr;
+    }
+    if (b->graph[slot]) {
+        (void)hipGraphDestroy(b->graph[slot]);
+        b->graph[slot] = nullptr;
+    }
+    if (!b->capture_stream) TF_HIP(hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking));
+    TF_HIP(hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_steps(b, steps, b->capture_stream, true);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->capture_stream, &g);
+    if (rc != TF_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(TF_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+    b->graph[slot] = g;
+    TF_HIP(hipGraphInstantiate(&b->exec[slot], g, nullptr, nullptr, 0));
+    b->graph_steps[slot] = steps;
+    return TF_OK;
+}
+
+// Steps run as replays of a graph of kGraphChunk steps plus one of the
+// remainder: instantiation cost stays that of at most 2 x kGraphChunk steps
+// whatever the step count, and one replay of 16 steps costs a single launch.
 int tf_batch_step_graph(tf_batch *b, int steps, void *stream) {
     TF_REQUIRE(b, "null batch");
     TF_REQUIRE(steps >= 1, "steps must be >= 1");
-    if (!b->exec || b->graph_steps != steps) {
-        if (b->exec) {
-            (void)hipGraphExecDestroy(b->exec);
-            b->exec = nullptr;
-        }
-        if (b->graph) {
-            (void)hipGraphDestroy(b->graph);
-            b->graph = nullptr;
-        }
-        if (!b->capture_stream) TF_HIP(hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking));
-        TF_HIP(hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal));
-        const int rc = enqueue_steps(b, steps, b->capture_stream, true);
-        hipGraph_t g = nullptr;
-        const hipError_t e = hipStreamEndCapture(b->capture_stream, &g);
-        if (rc != TF_OK) {
-            if (g) (void)hipGraphDestroy(g);
-            return rc;
-        }
-        if (e != hipSuccess) return fail(TF_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
-        b->graph = g;
-        TF_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
-        b->graph_steps = steps;
+    const int chunk = std::min(steps, (int)tf_batch::kGraphChunk);
+    if (int e = ensure_graph(b, 0, chunk)) return e;
+    for (int i = 0; i < steps / chunk; ++i) TF_HIP(hipGraphLaunch(b->exec[0], S_(stream)));
+    if (const int rem = steps % chunk) {
+        if (int e = ensure_graph(b, 1, rem)) return e;
+        TF_HIP(hipGraphLaunch(b->exec[1], S_(stream)));
     }
-    TF_HIP(hipGraphLaunch(b->exec, S_(stream)));
     return TF_OK;
 }
 
@@ -873,6 +934,12 @@ int tf_batch_step_timed(tf_batch *b, int steps, float *stencil_ms, void *stream)
         if (e) (void)hipEventDestroy(e);
     if (rc == TF_OK) *stencil_ms = total;
     return rc;
+}
+
+int tf_batch_reset(tf_batch *b, void *stream) {
+    TF_REQUIRE(b, "null batch");
+    TF_HIP(launch_reset_sims(b->d_sims, b->d_reset, b->d_speed_prefix, b->nsims, b->speed_blocks, S_(stream)));
+    return TF_OK;
 }
 
 // tf_batch_status copies the first sizeof(tf_sim_status) bytes of each Ctrl.

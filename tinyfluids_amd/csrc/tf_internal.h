@@ -42,7 +42,7 @@ struct SimDev {
     int waves;              // ntx * ntz * nyc
     int far;                // fields of exactly 4 GiB: tiles touching the last (x, y) row run in k_step_far
     int line_w;             // 0: halo tiling (waves of 62 z-columns); W >= 1: line tiling, W waves per z-line
-    int groups;             // line tiling: x-column groups per wave (nz <= 32 packs several)
+    int groups;             // line tiling: whole z-lines (x-column groups) per workgroup
 };
 
 // A tile handled by the far-row kernel: simulation index, x-tile, and the
@@ -58,6 +58,15 @@ struct FarList {
     int pad;
     FarTile t[kMaxFarPerSim];
 };
+
+// Loop bounds of one simulation of a plan (from its tf_sim_desc), for
+// tf_batch_reset.
+struct ResetParams {
+    double dx, c_cfl, t_final;
+    int64_t max_steps;
+};
+// Cells per workgroup of the batched signal-speed kernel (256 threads x 8).
+constexpr int kSpeedCellsPerBlock = 2048;
 
 // Workspace header of the single-grid entry points.
 // sim/prefix/ctrl: the single explicit-dt step through the whole-line

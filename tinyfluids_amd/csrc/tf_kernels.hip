@@ -356,7 +356,8 @@ __device__ __forceinline__ uint32_t fused_tile(const float *in_, float *out_, in
 
 // One workgroup of the line tiling (W waves, whole z-lines): returns the
 // wave-local max of the signal-speed bits of the cells this wave wrote.
-// `groups` x-column groups of nz lanes each (groups > 1 only when W == 1).
+// `groups` x-column groups of nz lanes each, lane after lane across the
+// workgroup's waves; lanes past groups * nz idle.
 template <int TX>
 __device__ __forceinline__ uint32_t fused_line(const float *in_, float *out_, int nx, int ny, int nz, int open_bc,
                                                const Gas &g_in, int W, int groups, int xt, int yc, int ly,
@@ -646,8 +647,7 @@ __global__ void k_finalize_one(Ctrl *C, int first) {
     if (threadIdx.x == 0 && blockIdx.x == 0) finalize_ctrl(C, first);
 }
 
-__global__ void k_ctrl_reset(Ctrl *C, double dx, double c_cfl, double t_final, int64_t max_steps) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__device__ __forceinline__ void ctrl_reset(Ctrl *C, double dx, double c_cfl, double t_final, int64_t max_steps) {
     C->t = 0.0;
     C->dt = 0.0;
     C->n_iter = 0;
@@ -661,6 +661,52 @@ __global__ void k_ctrl_reset(Ctrl *C, double dx, double c_cfl, double t_final, i
     C->dx = dx;
     C->c_cfl = c_cfl;
     C->max_steps = max_steps;
+}
+
+__global__ void k_ctrl_reset(Ctrl *C, double dx, double c_cfl, double t_final, int64_t max_steps) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) ctrl_reset(C, dx, c_cfl, t_final, max_steps);
+}
+
+// tf_batch_reset: every simulation's control block, then the signal-speed
+// maximum of every buf[0] (simulation s owns workgroups [bprefix[s],
+// bprefix[s+1]) of kSpeedCellsPerBlock cells), then the first dt.
+__global__ void k_reset_sims(const SimDev *__restrict__ sims, const ResetParams *__restrict__ rp, int nsims) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nsims) ctrl_reset(sims[i].ctrl, rp[i].dx, rp[i].c_cfl, rp[i].t_final, rp[i].max_steps);
+}
+
+__global__ __launch_bounds__(256) void k_speed_sims(const SimDev *__restrict__ sims, const int *__restrict__ bprefix,
+                                                    int nsims) {
+    const int b = (int)blockIdx.x;
+    if (b >= bprefix[nsims]) return;
+    int lo = 0, hi = nsims;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (bprefix[mid] <= b) lo = mid; else hi = mid;
+    }
+    const SimDev &S = sims[lo];
+    const int64_t nc = (int64_t)S.nx * S.ny * S.nz;
+    const float *prim = S.buf[0];
+    const int64_t base = (int64_t)(b - bprefix[lo]) * kSpeedCellsPerBlock;
+    uint32_t v = 0;
+#pragma unroll
+    for (int j = 0; j < kSpeedCellsPerBlock / 256; ++j) {
+        const int64_t i = base + j * 256 + threadIdx.x;
+        if (i < nc) {
+            const float r = prim[i], vx = prim[nc + i], vy = prim[2 * nc + i], vz = prim[3 * nc + i],
+                        p = prim[4 * nc + i];
+            const float c = fsqrt(S.gamma * p * frcp(r));
+            const float w[3] = {vx, vy, vz};
+            v = max(v, __float_as_uint(signal_speed(w, c)));
+        }
+    }
+    v = wave_umax(v);
+    if ((threadIdx.x & 63) == 0) atomicMax(&S.ctrl->max_bits[0], v);
+}
+
+__global__ void k_finalize_first_sims(const SimDev *__restrict__ sims, int nsims) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nsims) finalize_ctrl(sims[i].ctrl, 1);
 }
 
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, int n) {
@@ -936,6 +982,15 @@ hipError_t launch_direct_setup(const SimDev &S, SimDev *dst, int *prefix, int nb
 
 hipError_t launch_direct_bits(const Ctrl *C, uint32_t *bits_out, hipStream_t st) {
     hipLaunchKernelGGL(k_direct_bits, dim3(1), dim3(64), 0, st, C, bits_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_reset_sims(const SimDev *sims, const ResetParams *rp, const int *bprefix, int nsims,
+                             int speed_blocks, hipStream_t st) {
+    hipLaunchKernelGGL(k_reset_sims, grid1(nsims, 64), dim3(64), 0, st, sims, rp, nsims);
+    if (speed_blocks > 0)
+        hipLaunchKernelGGL(k_speed_sims, dim3((unsigned)speed_blocks), dim3(256), 0, st, sims, bprefix, nsims);
+    hipLaunchKernelGGL(k_finalize_first_sims, grid1(nsims, 64), dim3(64), 0, st, sims, nsims);
     return hipGetLastError();
 }
 

@@ -2,9 +2,9 @@
 job stream, each next to the plain path that runs one grid per launch
 sequence (the reference's own usage: one time_integration call per grid).
 
-- ``run_batched``: every simulation resident in HBM, all advanced together,
-  ``steps`` batched steps captured once into a HIP graph and replayed as one
-  launch.
+- ``run_batched``: every simulation resident in HBM, all reset together,
+  then advanced together: the batched steps replay captured HIP graphs of
+  16 steps.
 - ``run_plain``: the same simulations one after another through
   ``run_steps`` (the drop-in single-grid loop).
 - ``Server``: continuous batching. Jobs arrive over time; a job is admitted
@@ -67,8 +67,7 @@ def tune_batch(specs: Sequence[SimSpec], initial: Sequence[torch.Tensor], candid
     for tx in candidates:
         for ty in tile_y_candidates:
             b = Batch(sims, tile_x=tx, tile_y=ty)
-            for s in sims:
-                s.reset()
+            b.reset()
             b.step(1)  # warm-up
             results[(tx, ty)] = b.step_timed(steps) / steps
             b.close()
@@ -87,8 +86,7 @@ def run_batched(specs: Sequence[SimSpec], initial: Sequence[torch.Tensor], *, gr
     steps = max(s.steps for s in specs)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for s in sims:
-        s.reset()
+    batch.reset()
     if graph:
         batch.step_graph(steps)
     else:
