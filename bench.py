@@ -251,7 +251,9 @@ def plan_kernel(tx, waves_per_line):
         return f"tf::k_step_sims<{tx}>"
     if waves_per_line <= 4:
         return f"tf::k_line_sims<{tx}, {waves_per_line}>"
-    return "tf::k_line_wide<4, 8>" if tx == 4 else "tf::k_line_wide<2, 16>"
+    if waves_per_line <= 8:
+        return f"tf::k_line_wide<{tx}, 8>"
+    return "tf::k_line_wide<2, 16>"
 
 
 def measure_grid(dev, nx, ny, nz, steps, warmup, tile, no_tune, calibrate, copy_ceiling=False):

@@ -156,11 +156,17 @@ LINE_SHAPES = [((5, 6, 8), False), ((7, 5, 16), True), ((6, 4, 24), False), ((4,
 
 def halo_path_steps(x, dx, gamma, n, hb):
     """n steps through the single-step entry points: the CFL dt, then the
-    fused step, which always runs 62-cell z-tiles."""
+    fused step with the line tiling switched off (62-cell z-tiles)."""
     import tinyfluids_amd as tf
-    for _ in range(n):
-        dt = tf._cfl_time_step(x, dx, gamma)
-        x = tf._evolve_state(x, dx, dt, gamma, hb)
+    import tinyfluids_amd._lib as L
+    lib = L.load()
+    lib.tf_set_line_mode(0)
+    try:
+        for _ in range(n):
+            dt = tf._cfl_time_step(x, dx, gamma)
+            x = tf._evolve_state(x, dx, dt, gamma, hb)
+    finally:
+        lib.tf_set_line_mode(1)
     return x
 
 
@@ -168,8 +174,9 @@ def halo_path_steps(x, dx, gamma, n, hb):
 @pytest.mark.parametrize("tile_x", [2, 4, 8])
 def test_line_tiling_bitwise(line_mode, tile_x):
     """Batched plans with whole-z-line workgroups (W = 1..16 waves, several
-    x-groups per wave for nz <= 32) give the plain loop's results bit for
-    bit, and both equal the 62-cell z-tiling of the single-step path."""
+    lines per workgroup where that leaves fewer lanes idle) give the plain
+    loop's results bit for bit, and both equal the 62-cell z-tiling of the
+    single-step path."""
     import tinyfluids_amd._lib as L
     from tinyfluids_amd.batch import Batch, Simulation
     from tinyfluids_amd.time_integration import run_steps

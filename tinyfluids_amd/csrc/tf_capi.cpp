@@ -1057,7 +1057,7 @@ int tf_autotune_loop(float *state, float *scratch, int nx, int ny, int nz, doubl
     if (int e = tf_sim_reset(&d, stream)) return e;
     const double cells = (double)nx * ny * nz;
     const bool big = cells > 67108864.0;
-    std::vector<int> txs = big ? std::vector<int>{4, 8} : std::vector<int>{2, 4, 8};
+    const std::vector<int> txs{2, 4, 8};
     std::vector<int> lys = big ? std::vector<int>{16, 32, 64, 128, 256}
                                : std::vector<int>{0, 1, 2, 4, 8, 16, 32, 64, 128};
     hipEvent_t e0 = nullptr, e1 = nullptr;
