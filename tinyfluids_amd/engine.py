@@ -25,6 +25,7 @@ from __future__ import annotations
 
 import math
 import time
+from concurrent.futures import ThreadPoolExecutor, wait
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Sequence
 
@@ -186,7 +187,7 @@ class Server:
 
     def __init__(self, hbm_budget_bytes: int, max_resident: int = 1024, steps_per_tick: int = 0, device=None,
                  keep_states: bool = False, tile_x: int = 0, fill_cells: int = 1 << 24, tick_ms: float = 4.0,
-                 policy: str = "shortest"):
+                 policy: str = "shortest", prep_threads: int = 4):
         if policy not in ("shortest", "fcfs"):
             raise ValueError(f"policy must be 'shortest' or 'fcfs', got {policy!r}")
         self.policy = policy
@@ -198,6 +199,7 @@ class Server:
         self.tile_x = tile_x
         self.fill_cells = int(fill_cells)
         self.tick_ms = float(tick_ms)
+        self.prep_threads = max(1, int(prep_threads))
 
     @classmethod
     def group_of(cls, spec: SimSpec) -> int:
@@ -230,7 +232,11 @@ class Server:
         """Serves jobs arriving at spec.arrival * time_scale seconds after the
         start (wall clock). Returns per-job records. The tilings of the
         stream's grid shapes are measured beforehand, as serve_plain does; a
-        batch of one job then runs exactly like the plain path."""
+        batch of one job then runs exactly like the plain path.
+
+        A job's initial state is built by a pool of host threads from the
+        moment it arrives, and uploaded on a stream of its own when the job
+        is admitted, so neither waits for the device's ticks."""
         jobs = [JobRecord(i, s, s.arrival * time_scale) for i, s in enumerate(specs)]
         for j in jobs:
             if state_bytes(j.spec) > self.budget:
@@ -238,6 +244,13 @@ class Server:
         tune_plain(specs, self.device)
         ngroups = len(self.GROUP_BOUNDS) + 1
         streams = [torch.cuda.Stream(self.device) for _ in range(ngroups)]  # default priority
+        upload = torch.cuda.Stream(self.device)
+        with ThreadPoolExecutor(max_workers=self.prep_threads) as pool:
+            return self._serve(jobs, streams, upload, pool)
+
+    def _serve(self, jobs: List[JobRecord], streams, upload, pool) -> ServeReport:
+        ngroups = len(streams)
+        prepared = {}  # job index -> future of its initial state (host array)
         pending = list(jobs)  # in arrival order
         queue: List[JobRecord] = []
         active: Dict[int, Simulation] = {}
@@ -258,12 +271,21 @@ class Server:
             #    host prepares new jobs while the device runs the tick
             now = time.perf_counter() - t0
             while pending and pending[0].arrival <= now:
-                queue.append(pending.pop(0))
+                j = pending.pop(0)
+                queue.append(j)
+                prepared[j.index] = pool.submit(j.spec.initial_state)
             while queue and len(active) < self.max_resident and used + state_bytes(queue[0].spec) <= self.budget:
+                if not prepared[queue[0].index].done():
+                    break  # admission stays in arrival order
                 j = queue.pop(0)
                 s = j.spec
-                with torch.cuda.stream(streams[self.group_of(s)]):
-                    sim = Simulation(s.initial_state(), s.dx, s.gamma, max_steps=s.steps, device=self.device)
+                x = prepared.pop(j.index).result()
+                with torch.cuda.stream(upload):
+                    sim = Simulation(x, s.dx, s.gamma, max_steps=s.steps, device=self.device)
+                del x
+                g = self.group_of(s)
+                streams[g].wait_stream(upload)
+                with torch.cuda.stream(streams[g]):
                     sim.reset()
                 active[j.index] = sim
                 used += state_bytes(s)
@@ -302,9 +324,13 @@ class Server:
                         ids[g] = chosen
                         if chosen:
                             batches[g] = Batch([active[i] for i in chosen], tile_x=self.tile_x)
-            if all(b is None for b in batches) and pending and not queue:
-                # idle: wait for the next arrival
-                time.sleep(max(0.0, min(0.01, pending[0].arrival - (time.perf_counter() - t0))))
+            if all(b is None for b in batches):
+                if pending and not queue:
+                    # idle: wait for the next arrival
+                    time.sleep(max(0.0, min(0.01, pending[0].arrival - (time.perf_counter() - t0))))
+                elif queue and queue[0].index in prepared:
+                    # idle: wait for the oldest queued job's initial state
+                    wait([prepared[queue[0].index]], timeout=0.01)
         for g, b in enumerate(batches):
             if b is not None:
                 with torch.cuda.stream(streams[g]):

@@ -48,11 +48,14 @@ def blast(shape, dx=None, centre=(0.5, 0.5, 0.5), radius=0.1, p_in=10.0, p_out=1
     x = _centres(nx, dx)[:, None, None]
     y = _centres(ny, dx)[None, :, None]
     z = _centres(nz, dx)[None, None, :]
-    r2 = (x - centre[0]) ** 2 + (y - centre[1]) ** 2 + (z - centre[2]) ** 2
-    state = np.zeros((5, nx, ny, nz), dtype=np.float64)
-    state[0] = rho
-    state[4] = np.where(r2 < radius * radius, p_in, p_out)
-    return state.astype(dtype), dx
+    # the distance test in float64; the state is written in the output dtype
+    # directly (rho, p_in, p_out rounded once, as a float64 state cast would)
+    inside = ((x - centre[0]) ** 2 + (y - centre[1]) ** 2) + (z - centre[2]) ** 2 < radius * radius
+    state = np.zeros((5, nx, ny, nz), dtype=dtype)
+    state[0] = np.asarray(rho, dtype=np.float64).astype(dtype)
+    state[4] = np.where(inside, np.asarray(p_in, dtype=np.float64).astype(dtype),
+                        np.asarray(p_out, dtype=np.float64).astype(dtype))
+    return state, dx
 
 
 def blast_noise(n: int, seed: int = 1, dtype=np.float32):
