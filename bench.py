@@ -48,8 +48,6 @@ def parse():
     ap.add_argument("--no-harness", action="store_true")
     ap.add_argument("--harness-sizes", type=str, default="32,64,128,256,384,512,768")
     ap.add_argument("--no-shapes", action="store_true", help="skip the non-cubic 1536x512x512 roofline line")
-    ap.add_argument("--calibrate-copy", action="store_true",
-                    help="also copy the state buffer once (a plain copy kernel of known size, for HBM counters)")
     return ap.parse_args()
 
 
@@ -256,13 +254,14 @@ def plan_kernel(tx, waves_per_line):
     return "tf::k_line_wide<2, 16>"
 
 
-def measure_grid(dev, nx, ny, nz, steps, warmup, tile, no_tune, calibrate, copy_ceiling=False):
+def measure_grid(dev, nx, ny, nz, steps, warmup, tile, no_tune, copy_ceiling=False):
     """KH on nx x ny x nz resident in HBM: tiling (forced, measured by the
     loop tuner, or the default), `warmup` steps, then `steps` timed steps
     (wall clock and HIP events around the stencil launches). With
     copy_ceiling, also the rate of a flat copy of the state buffer
     (tf_hbm_copy: every byte read once and written once), the practical
-    ceiling of a read+write kernel on this device."""
+    ceiling of a read+write kernel on this device; its four dispatches of
+    known size also calibrate the HBM byte counters of a profiled run."""
     import ctypes
 
     import torch
@@ -286,10 +285,6 @@ def measure_grid(dev, nx, ny, nz, steps, warmup, tile, no_tune, calibrate, copy_
         _lib.check(lib.tf_set_tiling(nx, ny, nz, tx, ty))
     elif not no_tune:
         _, _, tuned_ms = autotune(buf0, dx, gamma, False, buf1)
-    if calibrate:
-        buf1.copy_(buf0)
-        torch.cuda.synchronize()
-        print(f"bench: calibration copy of {buf0.numel() * 4} bytes", file=sys.stderr, flush=True)
     copy_gbs = None
     if copy_ceiling:
         nbytes = buf0.numel() * 4
@@ -334,8 +329,7 @@ def main():
 
     nx, ny, nz = (int(v) for v in args.shape.split(","))
     dev = torch.device("cuda:0")
-    m = measure_grid(dev, nx, ny, nz, args.steps, args.warmup, args.tile, args.no_tune, args.calibrate_copy,
-                     copy_ceiling=True)
+    m = measure_grid(dev, nx, ny, nz, args.steps, args.warmup, args.tile, args.no_tune, copy_ceiling=True)
     tx, ly, st, el, stencil_ms = m["tile_x"], m["tile_y"], m["status"], m["seconds"], m["stencil_ms"]
     kernel = m["kernel"]
     cells = nx * ny * nz
@@ -370,7 +364,7 @@ def main():
         "sim_time": st.t,
     }
     if not args.no_shapes and (nx, ny, nz) == (1024, 1024, 1024):
-        m2 = measure_grid(dev, 1536, 512, 512, 20, 2, "", args.no_tune, False)
+        m2 = measure_grid(dev, 1536, 512, 512, 20, 2, "", args.no_tune)
         c2 = 1536 * 512 * 512
         k2 = m2["stencil_ms"] / 1e3 / 20
         rec["shapes"] = [{"grid": [1536, 512, 512], "workload": "Kelvin-Helmholtz, gamma 1.4, dx 1/512, 20 steps",
