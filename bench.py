@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--serve-rates", type=str, default="8,32,128", help="Poisson arrival rates (jobs/s) to sweep")
     ap.add_argument("--no-harness", action="store_true")
     ap.add_argument("--harness-sizes", type=str, default="32,64,128,256,384,512,768")
+    ap.add_argument("--figures", type=str, default="figures",
+                    help="directory for the harness's mid-plane density/pressure images ('' for none)")
     ap.add_argument("--no-shapes", action="store_true", help="skip the non-cubic 1536x512x512 roofline line")
     return ap.parse_args()
 
@@ -115,22 +117,30 @@ GH200_SECONDS = {32: 0.001980, 64: 0.006428, 96: 0.024395, 128: 0.082908, 196: 0
                  384: 7.514550, 512: 24.537891, 768: 128.716188}
 
 
-def harness_leg(dev, sizes):
+def harness_leg(dev, sizes, figures=""):
     """The reference's own benchmark (scaling_jax.py:42-73, 86-93): the blast
     cube setup_ics(N, N // 16), gamma 5/3, dx 1/(N-1), time_integration to
-    t = 0.2 without open boundaries; one warm-up call, then the minimum of 3
-    timed calls, each returning after the run (block_until_ready)."""
+    t = 0.2 without open boundaries; one warm-up call (whose result gives the
+    check figure and the blast-front radius), then the minimum of 3 timed
+    calls, each returning after the run (block_until_ready)."""
     import torch
 
     import tinyfluids_amd as tf
     from tinyfluids_amd import ics
+    from tinyfluids_amd.harness import plot_check
 
     out = []
     for n in sizes:
         s, dx = ics.setup_ics(n, n // 16)
         x = torch.from_numpy(s).to(dev)
         del s
-        _, iters = tf.time_integration(x, dx, 0.2, 5 / 3, False, False)
+        y, iters = tf.time_integration(x, dx, 0.2, 5 / 3, False, False)
+        c = n // 2
+        front = ics.front_radius_of_ray(y[4, c:, c, c].cpu().numpy(), dx)
+        if figures:
+            os.makedirs(figures, exist_ok=True)
+            plot_check(y[:, :, :, c].cpu().numpy(), os.path.join(figures, f"check_{n:d}.png"))
+        del y
         times = []
         for _ in range(3):
             torch.cuda.synchronize()
@@ -140,7 +150,8 @@ def harness_leg(dev, sizes):
             assert it == iters
             del y
         t = min(times)
-        row = {"N": n, "iterations": iters, "seconds": t, "cell_updates_per_s": n ** 3 * iters / t}
+        row = {"N": n, "iterations": iters, "seconds": t, "cell_updates_per_s": n ** 3 * iters / t,
+               "blast_front_radius": front}
         if n in GH200_SECONDS:
             row["gh200_seconds"] = GH200_SECONDS[n]
             row["speedup_vs_gh200"] = GH200_SECONDS[n] / t
@@ -374,7 +385,7 @@ def main():
         print(f"bench: shapes {rec['shapes']}", file=sys.stderr, flush=True)
     torch.cuda.empty_cache()
     if not args.no_harness:
-        rec["reference_harness"] = harness_leg(dev, [int(v) for v in args.harness_sizes.split(",")])
+        rec["reference_harness"] = harness_leg(dev, [int(v) for v in args.harness_sizes.split(",")], args.figures)
     if not args.no_batched:
         rec["batched"] = batched_leg(dev)
         print(f"bench: batched {rec['batched']}", file=sys.stderr, flush=True)

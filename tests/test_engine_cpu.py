@@ -57,6 +57,44 @@ def test_setup_ics_matches_reference_harness():
     assert int((s[0] == 1.0).sum()) == 4 ** 3
 
 
+def test_blast_front_of_reference_harness_problem():
+    """The oracle's run of the reference harness problem (setup_ics(64, 4),
+    gamma 5/3, t = 0.2) puts the pressure front at r ~ 0.38-0.39, where the
+    reference's own figures/check_{N}.png show it; the front helper reads a
+    hand-built profile exactly."""
+    from oracle import c_oracle
+    s, dx = ics.setup_ics(64, 4)
+    out, n = c_oracle.time_integration(s, dx, 0.2, 5 / 3, False)[:2]
+    assert n == 51
+    assert 0.37 <= ics.blast_front_radius(out, dx) <= 0.40
+    probe = np.zeros((5, 10, 10, 10), dtype=np.float32)
+    probe[4] = 0.1
+    probe[4, 5:8, 5, 5] = [2.0, 1.0, 0.9]  # midpoint 0.1 + (2.0 - 0.1) / 2 = 1.05: only cell c is above
+    assert ics.blast_front_radius(probe, 0.1) == 0.5 * 0.1
+    probe[4, 5:8, 5, 5] = [2.0, 1.5, 1.2]  # cells c .. c+2 are above
+    assert abs(ics.blast_front_radius(probe, 0.1) - 2.5 * 0.1) < 1e-12
+
+
+def test_oracle_run_renders_the_reference_check_figures():
+    """Golden outputs of the reference itself: its harness figures
+    figures/check_{32,64}.png (scaling_jax.py:75-82; tests/golden/
+    reference_check_figures.npz). The oracle's run of the same problem,
+    rendered the same way, matches them to within colormap quantisation:
+    max 16/255 per channel, mean 0.3/255 (measured 10 and 0.11 at N = 32,
+    3 and 0.005 at N = 64). A wrong sign, a dropped sweep or one step more
+    or less moves the blast and fails this."""
+    import os
+    from oracle import c_oracle
+    from tinyfluids_amd.harness import check_image
+    ref = np.load(os.path.join(os.path.dirname(__file__), "golden", "reference_check_figures.npz"))
+    for n in (32, 64):
+        s, dx = ics.setup_ics(n, n // 16)
+        out = c_oracle.time_integration(s, dx, 0.2, 5 / 3, False)[0]
+        img = check_image(out[:, :, :, n // 2])[::2, ::2].astype(np.int16)
+        d = np.abs(img - ref[f"check_{n}"].astype(np.int16))
+        assert d.max() <= 16 and d.mean() <= 0.3, (n, d.max(), d.mean())
+
+
 def _jobs(spec_list):
     from tinyfluids_amd.engine import JobRecord
     return [JobRecord(i, s, s.arrival) for i, s in enumerate(spec_list)]

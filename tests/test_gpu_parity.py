@@ -158,6 +158,32 @@ def test_iteration_counts(n, iters, hb):
     assert num == iters
 
 
+def test_harness_blast_front_at_reference_radius():
+    """The reference harness problem at N = 128 (setup_ics(128, 8), gamma
+    5/3, t = 0.2, 108 steps) on the GPU: the pressure front sits at
+    r ~ 0.38, as in the reference's figures/check_{N}.png."""
+    s, dx = ics.setup_ics(128, 8)
+    out, num = tf().time_integration(to_dev(s), dx, 0.2, 5 / 3, False, False)
+    assert num == 108
+    assert 0.37 <= ics.blast_front_radius(out.cpu().numpy(), dx) <= 0.40
+
+
+@pytest.mark.parametrize("n", [32, 64, 128])
+def test_harness_run_renders_the_reference_check_figure(n):
+    """The reference's own outputs: figures/check_{N}.png of its GH200 runs
+    (scaling_jax.py:75-82; tests/golden/reference_check_figures.npz). The
+    HIP run of the same problem, rendered the same way, matches them to
+    within colormap quantisation (max 16/255 per channel, mean 0.3/255)."""
+    import os
+    from tinyfluids_amd.harness import check_image
+    ref = np.load(os.path.join(os.path.dirname(__file__), "golden", "reference_check_figures.npz"))[f"check_{n}"]
+    s, dx = ics.setup_ics(n, n // 16)
+    out, _ = tf().time_integration(to_dev(s), dx, 0.2, 5 / 3, False, False)
+    img = check_image(out[:, :, :, n // 2].cpu().numpy())[::2, ::2].astype(np.int16)
+    d = np.abs(img - ref.astype(np.int16))
+    assert d.max() <= 16 and d.mean() <= 0.3, (n, d.max(), d.mean())
+
+
 def test_nan_ends_loop_like_reference():
     s, dx = ics.blast((16, 16, 16))
     s[4, 3, 4, 5] = -1.0  # negative pressure: sound speed NaN, dt NaN

@@ -38,6 +38,25 @@ def setup_ics(num_cells: int, num_injection_cells: int = 2, dtype=np.float32):
     return state.astype(dtype), grid_spacing
 
 
+def blast_front_radius(state, grid_spacing: float, p_background: float = 0.1) -> float:
+    """Radius of the pressure front of the setup_ics blast along +x through
+    the centre of the injection cube: the outermost cell whose pressure
+    exceeds the midpoint between the background and the peak on that ray.
+    The cube spans cells [c - k, c + k) with c = N // 2, so its centre sits
+    half a cell below cell c. At t = 0.2 the reference's figures show the
+    front at r ~ 0.38 (scaling_jax.py:75-82, figures/check_{N}.png)."""
+    c = state.shape[1] // 2
+    return front_radius_of_ray(state[4, c:, c, c], grid_spacing, p_background)
+
+
+def front_radius_of_ray(p_ray, grid_spacing: float, p_background: float = 0.1) -> float:
+    """blast_front_radius from the pressure ray p[4, c:, c, c] alone."""
+    p = np.asarray(p_ray, dtype=np.float64)
+    threshold = p_background + 0.5 * (p.max() - p_background)
+    i = int(np.nonzero(p > threshold)[0].max())
+    return (i + 0.5) * grid_spacing
+
+
 def blast(shape, dx=None, centre=(0.5, 0.5, 0.5), radius=0.1, p_in=10.0, p_out=1.0, rho=1.0,
           dtype=np.float32):
     """Spherical blast: rho, v=0, p_out background; p_in where |r - centre| < radius.
