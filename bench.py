@@ -46,7 +46,7 @@ def parse():
     ap.add_argument("--serve-jobs", type=int, default=40)
     ap.add_argument("--serve-rates", type=str, default="8,32,128", help="Poisson arrival rates (jobs/s) to sweep")
     ap.add_argument("--no-harness", action="store_true")
-    ap.add_argument("--harness-sizes", type=str, default="32,64,128,256,384,512,768")
+    ap.add_argument("--harness-sizes", type=str, default="32,64,96,128,196,256,384,512,768")
     ap.add_argument("--figures", type=str, default="figures",
                     help="directory for the harness's mid-plane density/pressure images ('' for none)")
     ap.add_argument("--no-shapes", action="store_true", help="skip the non-cubic 1536x512x512 roofline line")
