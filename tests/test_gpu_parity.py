@@ -103,6 +103,23 @@ def test_fused_step(shape, hb):
     assert np.all(rel_l1(out, ref, 5 / 3) < ONE_STEP_TOL)
 
 
+@pytest.mark.parametrize("shape", [(16, 16, 1), (16, 1, 16), (1, 16, 16), (1, 1, 64), (33, 1, 1), (1, 1, 1),
+                                   (2, 3, 1), (70, 1, 3)])
+def test_degenerate_axes(shape):
+    """Grids with extent 1 (2-D and 1-D problems, a single cell): a periodic
+    axis of one cell has zero flux difference, so the step equals the lower-
+    dimensional one; single steps and a 20-step run follow the oracle."""
+    s = random_state(shape, 8)
+    print(f"degenerate shape={shape}", flush=True)
+    out = tf()._evolve_state(to_dev(s), 0.05, 0.004, 5 / 3)
+    ref = npo._evolve_state(s.astype(np.float64), 0.05, 0.004, 5 / 3)
+    assert np.all(rel_l1(out, ref, 5 / 3) < ONE_STEP_TOL)
+    run, t, _ = tf().run_steps(to_dev(s), 0.05, 5 / 3, 20)
+    ref_run, t_ref, _ = c_oracle.run_steps(s.astype(np.float64), 0.05, 5 / 3, 20)
+    assert np.all(rel_l1(run, ref_run, 5 / 3) < RUN_TOL)
+    assert abs(t - t_ref) < 1e-5 * t_ref
+
+
 @pytest.mark.parametrize("tx", [2, 4, 8])
 def test_fused_tilings_agree_bitwise(tx):
     """Per-cell arithmetic does not depend on the tiling."""
