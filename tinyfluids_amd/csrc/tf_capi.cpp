@@ -1025,6 +1025,7 @@ int tf_time_integration(float *state, float *scratch, int nx, int ny, int nz, do
     const Ctrl *C = reinterpret_cast<const Ctrl *>(d.ctrl);
     Ctrl host;
     int64_t chunk = max_steps >= 0 && std::isinf(t_final) ? max_steps : 4;
+    int64_t last_n = -1;
     for (;;) {
         chunk = std::max<int64_t>(1, std::min<int64_t>(chunk, 4096));
         if (max_steps != 0)
@@ -1032,6 +1033,10 @@ int tf_time_integration(float *state, float *scratch, int nx, int ny, int nz, do
         TF_HIP(hipMemcpyAsync(&host, C, sizeof host, hipMemcpyDeviceToHost, st));
         TF_HIP(hipStreamSynchronize(st));
         if (!host.active) break;
+        // an active simulation advances by at least one step per chunk
+        if (host.n_iter == last_n)
+            return fail(TF_ERR_HIP, "time loop made no progress at step %lld", (long long)host.n_iter);
+        last_n = host.n_iter;
         int64_t est = max_steps >= 0 ? max_steps - host.n_iter : (int64_t)1 << 40;
         if (std::isfinite(t_final) && host.dt > 0.0)
             est = std::min<int64_t>(est, (int64_t)std::ceil(std::min((t_final - host.t) / host.dt, 1e9)) + 1);
