@@ -572,8 +572,16 @@ __device__ __forceinline__ void line_block(const SimDev *__restrict__ sims, cons
     const int xt = local % S.ntx, yc = local / S.ntx;
     uint32_t v = fused_line<TX>(S.buf[par], S.buf[par ^ 1], S.nx, S.ny, S.nz, S.open_bc, g, W, S.groups, xt, yc,
                                 S.ly, lq);
+    // one device-scope atomic per workgroup: atomics to the control line
+    // serialise across the XCDs (1% of a 1024^3 step with one per wave)
     v = wave_umax(v);
-    if (lane_id() == 0) atomicMax(&C->max_bits[par ^ 1], v);
+    __shared__ uint32_t wmax[16];
+    if (lane_id() == 0) wmax[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < W; ++i) v = max(v, wmax[i]);
+        atomicMax(&C->max_bits[par ^ 1], v);
+    }
 }
 
 // Line tiling, W <= 4 waves per workgroup (W fixed): block b of a class of
