@@ -552,7 +552,8 @@ size_t tf_batch_workspace_bytes(int nsims) {
     return align256((size_t)nsims * sizeof(SimDev)) + align256((size_t)(nsims + kClasses) * sizeof(int)) +
            align256((size_t)nsims * sizeof(Ctrl)) +
            align256((size_t)nsims * kMaxFarPerSim * kFarSplit * sizeof(FarTile)) +
-           align256((size_t)nsims * sizeof(ResetParams)) + align256((size_t)(nsims + 1) * sizeof(int));
+           align256((size_t)nsims * sizeof(ResetParams)) + align256((size_t)(nsims + 1) * sizeof(int)) +
+           align256((size_t)nsims * kSlotBytesPerSim);
 }
 
 
@@ -720,6 +721,9 @@ int tf_batch_create(tf_batch **out, const tf_sim_desc *sims, int nsims, void *wo
     b->d_reset = reinterpret_cast<ResetParams *>(p);
     p += align256((size_t)nsims * sizeof(ResetParams));
     b->d_speed_prefix = reinterpret_cast<int *>(p);
+    p += align256((size_t)(nsims + 1) * sizeof(int));
+    uint32_t *d_maxp = reinterpret_cast<uint32_t *>(p);
+    for (int k = 0; k < nsims; ++k) b->h_sims[k].maxp = d_maxp + (size_t)k * (kSlotBytesPerSim / sizeof(uint32_t));
     std::vector<ResetParams> rp(nsims);
     std::vector<int> sp(nsims + 1, 0);
     int64_t speed_blocks = 0;
@@ -744,7 +748,9 @@ int tf_batch_create(tf_batch **out, const tf_sim_desc *sims, int nsims, void *wo
     b->nlaunch = (b->nfar > 0 ? 1 : 0);
     for (int c = 0; c < kClasses; ++c) b->nlaunch += b->cls[c].count > 0 ? 1 : 0;
     hipStream_t st = S_(stream);
-    hipError_t e = hipMemcpyAsync(b->d_sims, b->h_sims.data(), nsims * sizeof(SimDev), hipMemcpyHostToDevice, st);
+    hipError_t e = hipMemsetAsync(d_maxp, 0, (size_t)nsims * kSlotBytesPerSim, st);  // not in a captured graph
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->d_sims, b->h_sims.data(), nsims * sizeof(SimDev), hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
         e = hipMemcpyAsync(b->d_prefix, b->h_prefix.data(), b->h_prefix.size() * sizeof(int), hipMemcpyHostToDevice,
                            st);

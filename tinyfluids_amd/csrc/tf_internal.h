@@ -43,7 +43,16 @@ struct SimDev {
     int far;                // fields of exactly 4 GiB: tiles touching the last (x, y) row run in k_step_far
     int line_w;             // 0: halo tiling (waves of 62 z-columns); W >= 1: line tiling, W waves per z-line
     int groups;             // line tiling: whole z-lines (x-column groups) per workgroup
+    int pad2;
+    // Signal-speed accumulators of a plan's step: per parity of the state
+    // written, kMaxSlots slots of one cache line each, one per XCD (block
+    // index mod 8); the dt update combines and clears them. nullptr: the
+    // step accumulates into ctrl->max_bits directly (single explicit step).
+    uint32_t *maxp;
 };
+constexpr int kMaxSlots = 8;
+constexpr int kSlotStride = 32;  // uint32 per slot: one 128-byte line
+constexpr size_t kSlotBytesPerSim = 2 * kMaxSlots * kSlotStride * sizeof(uint32_t);
 
 // A tile handled by the far-row kernel: simulation index, x-tile, and the
 // rows [y0, y0 + rows) it writes.

@@ -448,6 +448,19 @@ static inline unsigned xcd_grid(int total_waves) {
 template <int TX>
 constexpr int waves_per_simd() { return TX <= 2 ? 4 : (TX <= 4 ? 3 : (TX <= 6 ? 2 : 1)); }
 
+// A wave's or workgroup's signal-speed maximum of the state it wrote (the
+// step read parity par): into the plan's per-XCD slot of the simulation,
+// else into the control block. Device-scope atomics to one address
+// serialise across the XCDs (~20 ns each); one line per XCD keeps a step's
+// atomics from queueing on a single address.
+__device__ __forceinline__ void publish_max(const SimDev &S, Ctrl *C, int par, uint32_t v) {
+    uint32_t *maxp = S.maxp;
+    if (maxp)
+        atomicMax(maxp + ((par ^ 1) * kMaxSlots + (int)(blockIdx.x & (kMaxSlots - 1))) * kSlotStride, v);
+    else
+        atomicMax(&C->max_bits[par ^ 1], v);
+}
+
 // Does tile (xt, yc) load or store the last (x, y) row? Loaded x-rows are
 // x0-1 .. x0+TX (the first tile's left halo wraps to nx-1), loaded y-rows
 // y0-1 .. min(y0+ly, ny) (the first chunk's lower halo wraps to ny-1).
@@ -517,7 +530,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_pe
     uint32_t v = fused_tile<TX, false>(S.buf[par], S.buf[par ^ 1], S.nx, S.ny, S.nz, S.open_bc, g, xt, zt, y0,
                                        min(S.ly, S.ny - y0));
     v = wave_umax(v);
-    if (lane_id() == 0) atomicMax(&C->max_bits[par ^ 1], v);
+    if (lane_id() == 0) publish_max(S, C, par, v);
 }
 
 // The far tiles of k_step_sims: `far` lists (simulation, x-tile, row range);
@@ -538,7 +551,7 @@ __global__ __launch_bounds__(kBlock) void k_step_sims_far(const SimDev *__restri
     uint32_t v = fused_tile<TX, true>(S.buf[par], S.buf[par ^ 1], S.nx, S.ny, S.nz, S.open_bc, g, t.xt, zt, t.y0,
                                       t.rows);
     v = wave_umax(v);
-    if (lane_id() == 0) atomicMax(&C->max_bits[par ^ 1], v);
+    if (lane_id() == 0) publish_max(S, C, par, v);
 }
 
 // Line tiling: block b of a class of simulations with W waves per z-line;
@@ -580,7 +593,7 @@ __device__ __forceinline__ void line_block(const SimDev *__restrict__ sims, cons
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int i = 1; i < W; ++i) v = max(v, wmax[i]);
-        atomicMax(&C->max_bits[par ^ 1], v);
+        publish_max(S, C, par, v);
     }
 }
 
@@ -627,7 +640,21 @@ __device__ __forceinline__ void finalize_ctrl(Ctrl *C, int first) {
 
 __global__ void k_finalize_sims(const SimDev *__restrict__ sims, int nsims) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nsims) finalize_ctrl(sims[i].ctrl, 0);
+    if (i >= nsims) return;
+    const SimDev &S = sims[i];
+    Ctrl *C = S.ctrl;
+    if (S.maxp && C->active) {
+        // the step just run wrote parity p: combine and clear its slots
+        const int p = (int)((C->n_iter + 1) & 1);
+        uint32_t *slot = S.maxp + p * kMaxSlots * kSlotStride;
+        uint32_t bits = 0u;
+        for (int k = 0; k < kMaxSlots; ++k) {
+            bits = max(bits, slot[k * kSlotStride]);
+            slot[k * kSlotStride] = 0u;
+        }
+        C->max_bits[p] = bits;
+    }
+    finalize_ctrl(C, 0);
 }
 
 // Table and control block of a single explicit-dt step through the
