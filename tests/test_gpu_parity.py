@@ -120,6 +120,33 @@ def test_degenerate_axes(shape):
     assert abs(t - t_ref) < 1e-5 * t_ref
 
 
+@pytest.mark.parametrize("shape", [(12, 10, 70), (9, 8, 5), (6, 7, 200)])
+@pytest.mark.parametrize("with_workspace", [True, False])
+def test_single_step_reports_signal_speed_of_new_state(shape, with_workspace):
+    """tf_evolve_state's max_speed_out: the float bits of max |v_d| + c of the
+    state it wrote, equal to the CFL maximum of that state (whole-line path
+    with a workspace, 62-cell path without); the output does not depend on
+    whether the maximum is asked for."""
+    import ctypes
+    import tinyfluids_amd._lib as L
+    from tinyfluids_amd.fluid import workspace
+    lib = L.load()
+    s = to_dev(random_state(shape, 9))
+    dt = torch.tensor(0.003, dtype=torch.float64, device=DEV)
+    ws = workspace(*shape, DEV) if with_workspace else None
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    bits = torch.zeros(1, dtype=torch.int32, device=DEV)
+    out, out2 = torch.empty_like(s), torch.empty_like(s)
+    L.check(lib.tf_evolve_state(s.data_ptr(), out.data_ptr(), *shape, 0.05, dt.data_ptr(), 1.4, 0, bits.data_ptr(),
+                                ws.data_ptr() if ws is not None else None, st))
+    L.check(lib.tf_evolve_state(s.data_ptr(), out2.data_ptr(), *shape, 0.05, dt.data_ptr(), 1.4, 0, None,
+                                ws.data_ptr() if ws is not None else None, st))
+    torch.cuda.synchronize()
+    assert torch.equal(out, out2)
+    got = bits.view(torch.float32).item()
+    assert got == tf().max_signal_speed(out, 1.4).item()
+
+
 @pytest.mark.parametrize("tx", [2, 4, 8])
 def test_fused_tilings_agree_bitwise(tx):
     """Per-cell arithmetic does not depend on the tiling."""

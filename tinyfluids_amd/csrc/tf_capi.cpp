@@ -381,6 +381,7 @@ int tf_evolve_state(const float *in, float *out, int nx, int ny, int nz, double 
         S.ntx = (nx + groups * txl - 1) / (groups * txl);
         S.ntz = 1;
         S.waves = S.ntx * S.nyc * w;
+        S.skip_max = max_speed_out ? 0 : 1;
         const int nblocks = S.ntx * S.nyc;
         hipStream_t st = S_(stream);
         TF_HIP(launch_direct_setup(S, &H->sim, H->prefix, nblocks, dt, grid_spacing, st));

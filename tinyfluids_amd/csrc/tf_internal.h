@@ -43,7 +43,7 @@ struct SimDev {
     int far;                // fields of exactly 4 GiB: tiles touching the last (x, y) row run in k_step_far
     int line_w;             // 0: halo tiling (waves of 62 z-columns); W >= 1: line tiling, W waves per z-line
     int groups;             // line tiling: whole z-lines (x-column groups) per workgroup
-    int pad2;
+    int skip_max;           // 1: nobody reads this step's signal-speed maximum (single step without max_speed_out)
     // Signal-speed accumulators of a plan's step: per parity of the state
     // written, kMaxSlots slots of one cache line each, one per XCD (block
     // index mod 8); the dt update combines and clears them. nullptr: the

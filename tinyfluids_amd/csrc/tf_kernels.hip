@@ -448,12 +448,23 @@ static inline unsigned xcd_grid(int total_waves) {
 template <int TX>
 constexpr int waves_per_simd() { return TX <= 2 ? 4 : (TX <= 4 ? 3 : (TX <= 6 ? 2 : 1)); }
 
+// Maximum of v over a 256-thread block, valid in thread 0 (the block's
+// four waves meet in LDS: one atomic per block instead of per wave).
+__device__ __forceinline__ uint32_t block_umax256(uint32_t v) {
+    __shared__ uint32_t part[4];
+    v = wave_umax(v);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return max(max(part[0], part[1]), max(part[2], part[3]));
+}
+
 // A wave's or workgroup's signal-speed maximum of the state it wrote (the
 // step read parity par): into the plan's per-XCD slot of the simulation,
 // else into the control block. Device-scope atomics to one address
 // serialise across the XCDs (~20 ns each); one line per XCD keeps a step's
 // atomics from queueing on a single address.
 __device__ __forceinline__ void publish_max(const SimDev &S, Ctrl *C, int par, uint32_t v) {
+    if (S.skip_max) return;
     uint32_t *maxp = S.maxp;
     if (maxp)
         atomicMax(maxp + ((par ^ 1) * kMaxSlots + (int)(blockIdx.x & (kMaxSlots - 1))) * kSlotStride, v);
@@ -736,8 +747,8 @@ __global__ __launch_bounds__(256) void k_speed_sims(const SimDev *__restrict__ s
             v = max(v, __float_as_uint(signal_speed(w, c)));
         }
     }
-    v = wave_umax(v);
-    if ((threadIdx.x & 63) == 0) atomicMax(&S.ctrl->max_bits[0], v);
+    v = block_umax256(v);
+    if (threadIdx.x == 0) atomicMax(&S.ctrl->max_bits[0], v);
 }
 
 __global__ void k_finalize_first_sims(const SimDev *__restrict__ sims, int nsims) {
@@ -771,8 +782,8 @@ __global__ __launch_bounds__(256) void k_speed(const float *__restrict__ prim, i
         const float w[3] = {vx, vy, vz};
         v = max(v, __float_as_uint(signal_speed(w, c)));
     }
-    v = wave_umax(v);
-    if ((threadIdx.x & 63) == 0) atomicMax(bits_out, v);
+    v = block_umax256(v);
+    if (threadIdx.x == 0) atomicMax(bits_out, v);
 }
 
 __global__ void k_dt_from_bits(const uint32_t *bits, double dx, double c_cfl, double *dt_out, float *max_out) {
