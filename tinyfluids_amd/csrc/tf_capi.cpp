@@ -717,12 +717,13 @@ int tf_batch_create(tf_batch **out, const tf_sim_desc *sims, int nsims, void *wo
     p += align256((size_t)(nsims + kClasses) * sizeof(int));
     b->d_status = reinterpret_cast<Ctrl *>(p);
     p += align256((size_t)nsims * sizeof(Ctrl));
-    b->d_far = reinterpret_cast<FarTile *>(p);
-    p += align256((size_t)nsims * kMaxFarPerSim * kFarSplit * sizeof(FarTile));
     b->d_reset = reinterpret_cast<ResetParams *>(p);
     p += align256((size_t)nsims * sizeof(ResetParams));
     b->d_speed_prefix = reinterpret_cast<int *>(p);
     p += align256((size_t)(nsims + 1) * sizeof(int));
+    char *const tables_end = p;  // sims, prefix, status, reset, speed prefix: one upload
+    b->d_far = reinterpret_cast<FarTile *>(p);
+    p += align256((size_t)nsims * kMaxFarPerSim * kFarSplit * sizeof(FarTile));
     uint32_t *d_maxp = reinterpret_cast<uint32_t *>(p);
     for (int k = 0; k < nsims; ++k) b->h_sims[k].maxp = d_maxp + (size_t)k * (kSlotBytesPerSim / sizeof(uint32_t));
     std::vector<ResetParams> rp(nsims);
@@ -749,18 +750,23 @@ int tf_batch_create(tf_batch **out, const tf_sim_desc *sims, int nsims, void *wo
     b->nlaunch = (b->nfar > 0 ? 1 : 0);
     for (int c = 0; c < kClasses; ++c) b->nlaunch += b->cls[c].count > 0 ? 1 : 0;
     hipStream_t st = S_(stream);
+    // The tables from d_sims to the end of the speed prefix in one upload,
+    // through a host image of that workspace range (the status region in it
+    // is scratch); far tiles, when there are any, in a second; the
+    // signal-speed slots are cleared on the device.
+    char *const base = reinterpret_cast<char *>(b->d_sims);
+    std::vector<char> image(static_cast<size_t>(tables_end - base), 0);
+    auto put = [&](const void *dst, const void *src, size_t bytes) {
+        if (bytes) std::memcpy(image.data() + (static_cast<const char *>(dst) - base), src, bytes);
+    };
+    put(b->d_sims, b->h_sims.data(), nsims * sizeof(SimDev));
+    put(b->d_prefix, b->h_prefix.data(), b->h_prefix.size() * sizeof(int));
+    put(b->d_reset, rp.data(), rp.size() * sizeof(ResetParams));
+    put(b->d_speed_prefix, sp.data(), sp.size() * sizeof(int));
     hipError_t e = hipMemsetAsync(d_maxp, 0, (size_t)nsims * kSlotBytesPerSim, st);  // not in a captured graph
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(b->d_sims, b->h_sims.data(), nsims * sizeof(SimDev), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(b->d_prefix, b->h_prefix.data(), b->h_prefix.size() * sizeof(int), hipMemcpyHostToDevice,
-                           st);
+    if (e == hipSuccess) e = hipMemcpyAsync(base, image.data(), image.size(), hipMemcpyHostToDevice, st);
     if (e == hipSuccess && b->nfar > 0)
         e = hipMemcpyAsync(b->d_far, far.data(), far.size() * sizeof(FarTile), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(b->d_reset, rp.data(), rp.size() * sizeof(ResetParams), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(b->d_speed_prefix, sp.data(), sp.size() * sizeof(int), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
         delete b;
