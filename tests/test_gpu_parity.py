@@ -202,6 +202,34 @@ def test_iteration_counts(n, iters, hb):
     assert num == iters
 
 
+@pytest.mark.parametrize("hb", [False, True])
+@pytest.mark.parametrize("n", [96, 128, 196, 256])
+def test_iteration_counts_reference_sizes(n, hb):
+    """The reference's own step counts to t = 0.2 for setup_ics(N, N//16) at
+    every size its results/*.npz hold beyond 64 (tests/golden/
+    iteration_counts.json; the three files agree)."""
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "iteration_counts.json")) as f:
+        expected = json.load(f)["counts"]["scaling_data.npz"][str(n)]
+    s, dx = ics.setup_ics(n, n // 16)
+    _, num = tf().time_integration(to_dev(s), dx, 0.2, 5 / 3, hb, False)
+    assert num == expected
+
+
+def test_harness_problem_with_open_boundaries_meets_oracle():
+    """setup_ics(96, 6) to t = 0.2 with open boundaries (79 steps): relative L1
+    per conserved field within 1e-5 of the float64 oracle, same step count
+    and final time."""
+    s, dx = ics.setup_ics(96, 6)
+    out, num = tf().time_integration(to_dev(s), dx, 0.2, 5 / 3, True, False)
+    ref, n_ref, t_ref = c_oracle.time_integration(s.astype(np.float64), dx, 0.2, 5 / 3, True)[:3]
+    assert num == n_ref == 79
+    err = rel_l1(out, ref, 5 / 3)
+    print("setup_ics(96) open boundaries rel L1", err, flush=True)
+    assert np.all(err < RUN_TOL)
+
+
 def test_harness_blast_front_at_reference_radius():
     """The reference harness problem at N = 128 (setup_ics(128, 8), gamma
     5/3, t = 0.2, 108 steps) on the GPU: the pressure front sits at
