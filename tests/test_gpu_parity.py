@@ -240,7 +240,7 @@ def test_harness_blast_front_at_reference_radius():
     assert 0.37 <= ics.blast_front_radius(out.cpu().numpy(), dx) <= 0.40
 
 
-@pytest.mark.parametrize("n", [32, 64, 128])
+@pytest.mark.parametrize("n", [32, 64, 128, 256, 512])
 def test_harness_run_renders_the_reference_check_figure(n):
     """The reference's own outputs: figures/check_{N}.png of its GH200 runs
     (scaling_jax.py:75-82; tests/golden/reference_check_figures.npz). The
