@@ -34,6 +34,36 @@ def test_library_exports_every_declared_symbol():
     assert set(declared_functions()) <= set(_lib.SIGNATURES)
 
 
+def test_header_is_plain_c_and_links(tmp_path):
+    """include/tinyfluids.h compiles as C99 with warnings as errors, a C
+    program that takes the address of every declared entry point links
+    against libtfhip.so, and it runs without a GPU for the calls that need
+    none (version, error string, argument checks)."""
+    import shutil
+    import subprocess
+    from tinyfluids_amd import _lib
+    if shutil.which("gcc") is None:
+        pytest.skip("no C compiler")
+    lib_path = _lib.load()._name
+    names = declared_functions()
+    src = tmp_path / "abi_check.c"
+    src.write_text(
+        '#include <stdio.h>\n#include "tinyfluids.h"\n'
+        "typedef void (*fn_t)(void);\n"
+        "static fn_t table[] = {" + ", ".join(f"(fn_t){n}" for n in names) + "};\n"
+        "int main(void) {\n"
+        "    if (tf_abi_version() != TF_ABI_VERSION) return 1;\n"
+        "    if (tf_evolve_state(0, 0, 4, 4, 4, 0.1, 0, 1.4, 0, 0, 0, 0) != TF_ERR_ARG) return 2;\n"
+        "    printf(\"%d %s\\n\", (int)(sizeof table / sizeof table[0]), tf_last_error());\n"
+        "    return 0;\n}\n")
+    exe = tmp_path / "abi_check"
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-Werror", "-pedantic", f"-I{os.path.join(ROOT, 'include')}",
+                    str(src), lib_path, f"-Wl,-rpath,{os.path.dirname(lib_path)}", "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    assert int(out[0]) == len(names)
+    assert "null" in " ".join(out[1:])
+
+
 def test_argument_errors_without_gpu():
     from tinyfluids_amd import _lib
     lib = _lib.load()
