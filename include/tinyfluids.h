@@ -66,11 +66,14 @@ int tf_primitive_state_from_conserved(const float *cons, float *prim, int64_t nc
 int tf_cfl_time_step(const float *prim, int nx, int ny, int nz, double grid_spacing, double gamma, double c_cfl,
                      double *dt_out, float *max_speed_out, void *workspace, void *stream);
 
-/* Replaces _open_boundaries(state, 1) (fluid.py:382-413): out = in with the
- * first and last cell along every axis copied from their inner neighbours,
- * i.e. out[v][x][y][z] = in[v][g(x)][g(y)][g(z)], g = clamp(., 1, n-2).
- * Requires every n >= 2. in and out must not alias. */
-int tf_open_boundaries(const float *in, float *out, int nx, int ny, int nz, void *stream);
+/* Replaces _open_boundaries(state, num_ghost_cells) (fluid.py:317-413): along
+ * x, then y, then z, the first g cells take the value of cell g and then the
+ * last g cells that of cell n-g-1, so out[v][x][y][z] =
+ * in[v][s(x)][s(y)][s(z)] with s(i) = L(i >= n-g ? n-g-1 : i),
+ * L(j) = max(j, g); for g = 1 (what _evolve_state uses, fluid.py:572) s is
+ * clamp(., 1, n-2). Requires g >= 0 and every n >= g + 1. in and out must
+ * not alias. */
+int tf_open_boundaries(const float *in, float *out, int nx, int ny, int nz, int num_ghost_cells, void *stream);
 
 /* Replaces _evolve_state_along_axis (fluid.py:506-557): one split sweep along
  * axis (1 = x, 2 = y, 3 = z) with the HLL flux at every periodic interface:

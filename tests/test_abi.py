@@ -73,7 +73,10 @@ def test_argument_errors_without_gpu():
     assert lib.tf_evolve_state_along_axis(ctypes.c_void_p(16), ctypes.c_void_p(32), 4, 4, 4, 0.1,
                                           ctypes.c_void_p(48), 1.4, 7, None) == _lib.TF_ERR_ARG
     assert b"axis" in lib.tf_last_error()
-    assert lib.tf_open_boundaries(ctypes.c_void_p(16), ctypes.c_void_p(16), 4, 4, 4, None) == _lib.TF_ERR_ARG
+    assert lib.tf_open_boundaries(ctypes.c_void_p(16), ctypes.c_void_p(16), 4, 4, 4, 1, None) == _lib.TF_ERR_ARG
+    assert lib.tf_open_boundaries(ctypes.c_void_p(16), ctypes.c_void_p(32), 4, 4, 4, 4, None) == _lib.TF_ERR_ARG
+    assert b"ghost" in lib.tf_last_error()
+    assert lib.tf_open_boundaries(ctypes.c_void_p(16), ctypes.c_void_p(32), 4, 4, 4, -1, None) == _lib.TF_ERR_ARG
     assert lib.tf_evolve_state(ctypes.c_void_p(16), ctypes.c_void_p(32), 0, 4, 4, 0.1, ctypes.c_void_p(48), 1.4, 0,
                                None, None, None) == _lib.TF_ERR_ARG
     assert lib.tf_set_tiling(8, 8, 8, 3, 8) == _lib.TF_ERR_ARG

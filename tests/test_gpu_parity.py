@@ -175,6 +175,19 @@ def test_open_boundaries():
     assert np.array_equal(out.cpu().numpy(), ref)
 
 
+@pytest.mark.parametrize("g", [0, 1, 2, 3])
+@pytest.mark.parametrize("shape", [(9, 8, 7), (4, 5, 6), (7, 4, 12)])
+def test_open_boundaries_ghost_cells(g, shape):
+    """Any ghost-cell count equals the reference's sequential left/right copies
+    (restated in the NumPy oracle), down to dimensions of g + 1."""
+    if min(shape) < g + 1:
+        pytest.skip("dimension below g + 1")
+    s = random_state(shape, 10 + g)
+    out = tf()._open_boundaries(to_dev(s), g)
+    ref = npo._open_boundaries(s.copy(), g)
+    assert np.array_equal(out.cpu().numpy(), ref)
+
+
 def test_blast_64_100_steps():
     """64^3 periodic spherical blast, gamma 5/3, CFL 0.4, 100 steps."""
     s0, dx = ics.blast((64, 64, 64))

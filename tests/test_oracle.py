@@ -130,6 +130,20 @@ def test_open_boundaries_equal_index_clamp():
         assert np.array_equal(c_oracle.open_boundaries(s), ref)
 
 
+def test_open_boundaries_any_ghost_count_equal_index_map():
+    """For g ghost cells the reference's sequential copies (left then right,
+    x then y then z) equal the separable index map s(i) = max(j, g) with
+    j = n-g-1 for i >= n-g, else i: brute force on small grids, down to
+    n = g + 1 where the two sides overlap."""
+    for g in range(0, 4):
+        for shape in [(9, 8, 7), (g + 1, g + 2, 2 * g + 1), (2 * g + 2, g + 1, 6)]:
+            s = rand_state(shape, 7 + g)
+            out = npo._open_boundaries(s.copy(), g)
+            idx = [np.array([max(n - g - 1 if i >= n - g else i, g) for i in range(n)]) for n in shape]
+            ref = s[:, idx[0]][:, :, idx[1]][:, :, :, idx[2]]
+            assert np.array_equal(out, ref), (g, shape)
+
+
 # ---------------------------------------------------------------- invariants
 def test_uniform_state_is_steady():
     s = np.empty((5, 4, 5, 6))

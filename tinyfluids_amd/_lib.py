@@ -60,7 +60,7 @@ SIGNATURES = {
     "tf_conserved_state_from_primitive": (I, [P, P, I64, D, P]),
     "tf_primitive_state_from_conserved": (I, [P, P, I64, D, P]),
     "tf_cfl_time_step": (I, [P, I, I, I, D, D, D, P, P, P, P]),
-    "tf_open_boundaries": (I, [P, P, I, I, I, P]),
+    "tf_open_boundaries": (I, [P, P, I, I, I, I, P]),
     "tf_evolve_state_along_axis": (I, [P, P, I, I, I, D, P, D, I, P]),
     "tf_evolve_state": (I, [P, P, I, I, I, D, P, D, I, P, P, P]),
     "tf_time_integration": (I, [P, P, I, I, I, D, D, D, I, I64, D, P,

@@ -44,7 +44,7 @@ hipError_t launch_dt_from_bits(const uint32_t *bits, double dx, double c_cfl, do
                                hipStream_t st);
 hipError_t launch_sweep(int axis, const float *in, float *out, int nx, int ny, int nz, const Gas &g, const double *dt,
                         double dx, hipStream_t st);
-hipError_t launch_open_bc(const float *in, float *out, int nx, int ny, int nz, hipStream_t st);
+hipError_t launch_open_bc(const float *in, float *out, int nx, int ny, int nz, int g, hipStream_t st);
 hipError_t launch_p2c(const float *in, float *out, int64_t nc, const Gas &g, hipStream_t st);
 hipError_t launch_c2p(const float *in, float *out, int64_t nc, const Gas &g, hipStream_t st);
 hipError_t launch_totals(const float *prim, int64_t nc, double gamma, double *partials, double *out5,
@@ -333,12 +333,14 @@ int tf_cfl_time_step(const float *prim, int nx, int ny, int nz, double grid_spac
     return TF_OK;
 }
 
-int tf_open_boundaries(const float *in, float *out, int nx, int ny, int nz, void *stream) {
+int tf_open_boundaries(const float *in, float *out, int nx, int ny, int nz, int num_ghost_cells, void *stream) {
     if (int e = check_dims(nx, ny, nz)) return e;
     TF_REQUIRE(in && out, "null state pointer");
     TF_REQUIRE(in != out, "in and out must not alias");
-    if (int e = check_open_bc(1, nx, ny, nz)) return e;
-    TF_HIP(launch_open_bc(in, out, nx, ny, nz, S_(stream)));
+    TF_REQUIRE(num_ghost_cells >= 0, "num_ghost_cells must be >= 0 (got %d)", num_ghost_cells);
+    TF_REQUIRE(nx > num_ghost_cells && ny > num_ghost_cells && nz > num_ghost_cells,
+               "open boundaries with %d ghost cells need every dimension >= %d", num_ghost_cells, num_ghost_cells + 1);
+    TF_HIP(launch_open_bc(in, out, nx, ny, nz, num_ghost_cells, S_(stream)));
     return TF_OK;
 }
 

@@ -844,8 +844,17 @@ __global__ __launch_bounds__(256) void k_sweep(const float *__restrict__ in, flo
 }
 
 // ---------------------------------------------------------------- elementwise
+// Source index along one axis of the open boundaries with g ghost cells
+// (fluid.py:334-380): the left copies (cells [0, g) take cell g) run first,
+// then the right ones (cells [n-g, n) take cell n-g-1, as the left copies
+// left it). Needs n >= g + 1; g = 1 is open_clamp.
+__device__ __forceinline__ int open_src(int i, int n, int g) {
+    const int j = i >= n - g ? n - g - 1 : i;
+    return j < g ? g : j;
+}
+
 __global__ __launch_bounds__(256) void k_open_bc(const float *__restrict__ in, float *__restrict__ out, int nx, int ny,
-                                                 int nz) {
+                                                 int nz, int g) {
     const int64_t nc = (int64_t)nx * ny * nz;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nc) return;
@@ -853,7 +862,7 @@ __global__ __launch_bounds__(256) void k_open_bc(const float *__restrict__ in, f
     const int64_t xy = i / nz;
     const int y = (int)(xy % ny);
     const int x = (int)(xy / ny);
-    const int64_t src = ((int64_t)open_clamp(x, nx) * ny + open_clamp(y, ny)) * nz + open_clamp(z, nz);
+    const int64_t src = ((int64_t)open_src(x, nx, g) * ny + open_src(y, ny, g)) * nz + open_src(z, nz, g);
 #pragma unroll
     for (int f = 0; f < 5; ++f) out[f * nc + i] = in[f * nc + src];
 }
@@ -1101,9 +1110,9 @@ hipError_t launch_sweep(int axis, const float *in, float *out, int nx, int ny, i
     return hipGetLastError();
 }
 
-hipError_t launch_open_bc(const float *in, float *out, int nx, int ny, int nz, hipStream_t st) {
+hipError_t launch_open_bc(const float *in, float *out, int nx, int ny, int nz, int g, hipStream_t st) {
     const int64_t nc = (int64_t)nx * ny * nz;
-    hipLaunchKernelGGL(k_open_bc, grid1(nc), dim3(256), 0, st, in, out, nx, ny, nz);
+    hipLaunchKernelGGL(k_open_bc, grid1(nc), dim3(256), 0, st, in, out, nx, ny, nz, g);
     return hipGetLastError();
 }
 

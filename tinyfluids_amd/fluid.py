@@ -121,13 +121,13 @@ def max_signal_speed(primitive_state, gamma):
 
 
 def _open_boundaries(primitive_state, num_ghost_cells=1):
-    """fluid.py:382-413 (the reference step uses one ghost cell, :572)."""
-    if num_ghost_cells != 1:
-        raise NotImplementedError("only num_ghost_cells=1 (what _evolve_state uses) is implemented")
+    """fluid.py:317-413: open boundaries with num_ghost_cells ghost cells per
+    side (the reference step uses one, :572)."""
     s = check_state(primitive_state)
     _, nx, ny, nz = s.shape
     out = torch.empty_like(s)
-    _lib.check(_lib.load().tf_open_boundaries(_ptr(s), _ptr(out), nx, ny, nz, _stream(s.device)))
+    _lib.check(_lib.load().tf_open_boundaries(_ptr(s), _ptr(out), nx, ny, nz, int(num_ghost_cells),
+                                              _stream(s.device)))
     return out
 
 
