@@ -9,11 +9,13 @@ is the one the loop tuner measures fastest (for 1024^3: whole z-lines, 16
 waves per workgroup).
 
 Prints one JSON line: cell-updates/s, the HBM roofline of the stencil kernel
-(40 algorithmic bytes per cell-update: 5 float32 read + 5 written), and the
-CPU restatement of the reference algorithm (C, float32, OpenMP) timed on a
-bounded z-slab of the same workload; further legs: the non-cubic grid, the
-reference's own harness against its published GH200 times, the uneven batch
-and the served job stream against the plain path.
+(40 algorithmic bytes per cell-update: 5 float32 read + 5 written) beside the
+measured flat-copy ceiling of the box and the committed counter profiles,
+and the CPU restatement of the reference algorithm (C, float32, OpenMP)
+timed on a bounded z-slab of the same workload; further legs: the non-cubic
+grid, the reference's own harness against its published GH200 times (with
+its mid-plane check figures written to --figures), the uneven batch and the
+served job stream against the plain path.
 """
 
 from __future__ import annotations
