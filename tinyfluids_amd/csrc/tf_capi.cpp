@@ -82,6 +82,19 @@ int fail(int code, const char *fmt, ...) {
 
 inline hipStream_t S_(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
+// The two timing events of a tuner, destroyed on every return path.
+struct EventPair {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t create() {
+        const hipError_t e = hipEventCreate(&e0);
+        return e == hipSuccess ? hipEventCreate(&e1) : e;
+    }
+    ~EventPair() {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
+};
+
 int check_dims(int nx, int ny, int nz) {
     TF_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1, "grid dimensions must be >= 1 (got %d x %d x %d)", nx, ny, nz);
     const double cells = (double)nx * ny * nz;
@@ -454,9 +467,9 @@ int tf_autotune_evolve(const float *in, float *out, int nx, int ny, int nz, doub
     std::vector<int> lys;
     for (int v : {8, 16, 32, 64, 128, 256, 512}) if (v < ny) lys.push_back(v);
     lys.push_back(ny);
-    hipEvent_t e0, e1;
-    TF_HIP(hipEventCreate(&e0));
-    TF_HIP(hipEventCreate(&e1));
+    EventPair ev;
+    TF_HIP(ev.create());
+    hipEvent_t e0 = ev.e0, e1 = ev.e1;
     float best = 1e30f;
     int btx = 4, bly = ny;
     int rc = TF_OK;
@@ -490,8 +503,6 @@ int tf_autotune_evolve(const float *in, float *out, int nx, int ny, int nz, doub
         }
         if (rc != TF_OK) break;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     if (rc != TF_OK) return rc;
     {
         std::lock_guard<std::mutex> lk(g_tiling_mu);
@@ -1080,9 +1091,9 @@ int tf_autotune_loop(float *state, float *scratch, int nx, int ny, int nz, doubl
     const std::vector<int> txs{2, 4, 8};
     std::vector<int> lys = big ? std::vector<int>{16, 32, 64, 128, 256}
                                : std::vector<int>{0, 1, 2, 4, 8, 16, 32, 64, 128};
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    TF_HIP(hipEventCreate(&e0));
-    TF_HIP(hipEventCreate(&e1));
+    EventPair ev;
+    TF_HIP(ev.create());
+    hipEvent_t e0 = ev.e0, e1 = ev.e1;
     float best = 1e30f;
     int btx = default_loop_tx(nx, ny, nz), bly = 0, bline = 1;
     std::vector<std::tuple<int, int, int>> seen;  // (tx, resolved ly, line waves)
@@ -1127,8 +1138,6 @@ int tf_autotune_loop(float *state, float *scratch, int nx, int ny, int nz, doubl
         }
         if (rc != TF_OK) break;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     if (rc != TF_OK) return rc;
     {
         std::lock_guard<std::mutex> lk(g_tiling_mu);
