@@ -205,8 +205,8 @@ def measured_valu(nx, ny, nz, kernel, tile):
 
 def batched_leg(dev):
     """256 uneven blasts, sides {32,48,64,96,128}^3, 100 steps each: all
-    advanced together by one captured HIP graph, against the plain path that
-    runs them one after another through run_steps."""
+    advanced together by replays of captured HIP graphs, against the plain
+    path that runs them one after another through run_steps."""
     import torch
 
     from tinyfluids_amd import engine, ics
@@ -228,7 +228,8 @@ def batched_leg(dev):
         "speedup": p.seconds / b.seconds, "cell_updates_per_s": cells / b.seconds,
         "plain_cell_updates_per_s": cells / p.seconds, "bitwise_identical_to_plain": identical,
         "tile_x": tuned["tile_x"], "autotune_ms_per_step": tuned["candidates"],
-        "launch": "one HIP graph of 100 steps (stencil launches per tiling class + per-simulation dt update)",
+        "launch": "a captured HIP graph of 16 steps replayed 6 times, then one of 4 steps (each step: stencil "
+                  "launches per tiling class + per-simulation dt update)",
     }
 
 
