@@ -27,7 +27,8 @@
 // Cache policy of the state stores: plain write-back stores, so that L2
 // merges the partial 128-byte lines that two neighbouring 62-cell z-tiles
 // write; streaming (NT) stores write those halves to HBM separately and
-// measured ~25% slower.
+// measured ~25% slower. Whole-line tiles write full lines, yet NT stores
+// measured 4% slower there too at 1024^3 (and 0.5% faster at 1536x512x512).
 constexpr int kStoreAux = 0;
 
 namespace tf {
