@@ -269,6 +269,46 @@ def test_harness_run_renders_the_reference_check_figure(n):
     assert d.max() <= 16 and d.mean() <= 0.3, (n, d.max(), d.mean())
 
 
+def test_concurrent_callers_on_their_own_streams():
+    """Threads calling the drop-in loop at once, each on its own torch stream
+    and with shapes tuned concurrently on first use, get bit for bit what the
+    same calls give one after another (the calls share no device state; the
+    tiling registry is locked)."""
+    import threading
+
+    cases = [((40, 36, 44), 7, None), ((52, 48, 20), 8, None), ((40, 36, 44), 9, None), ((48, 48, 48), 0, 3)]
+
+    def run(shape, seed, n):
+        if n is None:
+            out, _, _ = tf().run_steps(to_dev(random_state(shape, seed)), 1.0 / shape[1], 1.4, 25, seed % 2 == 1)
+            return out, 25
+        s0, dx = ics.setup_ics(shape[0], n)
+        return tf().time_integration(to_dev(s0), dx, 0.2, 5 / 3, True, False)
+
+    results = [None] * len(cases)
+    errors = []
+
+    def worker(k):
+        try:
+            st = torch.cuda.Stream(DEV)
+            with torch.cuda.stream(st):
+                out, num = run(*cases[k])
+                results[k] = (out.cpu(), num)
+        except Exception as e:  # reported below
+            errors.append((k, repr(e)))
+
+    threads = [threading.Thread(target=worker, args=(k,)) for k in range(len(cases))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors
+    for k, case in enumerate(cases):
+        out, num = run(*case)
+        assert num == results[k][1], case
+        assert torch.equal(out.cpu(), results[k][0]), case
+
+
 def test_nan_ends_loop_like_reference():
     s, dx = ics.blast((16, 16, 16))
     s[4, 3, 4, 5] = -1.0  # negative pressure: sound speed NaN, dt NaN
