@@ -81,6 +81,15 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t field_rsrc_indexed(const float
 // lane-masked block. Emits no instruction.
 __device__ __forceinline__ void pin_vgpr(float &v) { asm volatile("" : "+v"(v)); }
 
+// Raised wave priority while a wave issues a column's loads or stores. The
+// SIMD's arbiter then issues them ahead of the other resident waves'
+// arithmetic, so more requests are in flight: 4.5% on the 1024^3 step and
+// 2-3% on 512-cell lines, measured (levels 1-3 alike, stores included best).
+struct MemPriority {
+    __device__ __forceinline__ MemPriority() { __builtin_amdgcn_s_setprio(2); }
+    __device__ __forceinline__ ~MemPriority() { __builtin_amdgcn_s_setprio(0); }
+};
+
 __device__ __forceinline__ float sgpr_f(float v) {
     return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
 }
@@ -114,6 +123,7 @@ struct HaloCtx {
 
     __device__ __forceinline__ void load(int j, uint32_t yoff, float (&w)[5]) const {
         const uint32_t soff = xoff[j] + yoff;
+        const MemPriority prio;
         if constexpr (FAR) {
 #pragma unroll
             for (int f = 0; f < 5; ++f) w[f] = in_ptr[f * nc + ((size_t)soff + zb) / 4];
@@ -143,6 +153,7 @@ struct HaloCtx {
     __device__ __forceinline__ bool out_ok(int s) const { return zout && s < nxo; }
     __device__ __forceinline__ void store(int s, int y, const float (&w)[5]) const {
         const uint32_t so = (uint32_t)(x0 + s) * plane_b + (uint32_t)y * row_b + 4u * (uint32_t)z0;
+        const MemPriority prio;
         if constexpr (FAR) {
 #pragma unroll
             for (int f = 0; f < 5; ++f) out_ptr[f * nc + ((size_t)so + zbo) / 4] = w[f];
@@ -179,6 +190,7 @@ struct LineCtx {
     static constexpr int kLq = 13;
 
     __device__ __forceinline__ void load(int j, uint32_t yoff, float (&w)[5]) const {
+        const MemPriority prio;
 #pragma unroll
         for (int f = 0; f < 5; ++f) w[f] = tf_struct_load_f32(rin[f], (int)xi[j], 0, (int)yoff, 0);
     }
@@ -215,6 +227,7 @@ struct LineCtx {
     __device__ __forceinline__ bool out_ok(int s) const { return valid && xs0 + s < nx; }
     __device__ __forceinline__ void store(int s, int y, const float (&w)[5]) const {
         const uint32_t so = (uint32_t)y * row_b + (uint32_t)s * plane_b;
+        const MemPriority prio;
 #pragma unroll
         for (int f = 0; f < 5; ++f) tf_struct_store_f32(w[f], rout[f], (int)xs_i, 0, (int)so, kStoreAux);
     }
