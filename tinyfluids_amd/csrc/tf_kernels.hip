@@ -83,8 +83,9 @@ __device__ __forceinline__ void pin_vgpr(float &v) { asm volatile("" : "+v"(v));
 
 // Raised wave priority while a wave issues a column's loads or stores. The
 // SIMD's arbiter then issues them ahead of the other resident waves'
-// arithmetic, so more requests are in flight: 4.5% on the 1024^3 step and
-// 2-3% on 512-cell lines, measured (levels 1-3 alike, stores included best).
+// arithmetic, so more requests are in flight. Measured: 4.4% faster on the
+// 1024^3 step, 2-3% on 256- and 512-cell lines, neutral on 62-cell tiles;
+// levels 1-3 alike, loads and stores together best.
 struct MemPriority {
     __device__ __forceinline__ MemPriority() { __builtin_amdgcn_s_setprio(2); }
     __device__ __forceinline__ ~MemPriority() { __builtin_amdgcn_s_setprio(0); }
