@@ -187,8 +187,8 @@ struct LineCtx {
     int nx;
     int lanes;            // 64 W: LDS stride of one quantity
     bool valid;
-    float *lq;            // LDS: 13 x 64W floats per slot parity
-    static constexpr int kLq = 13;
+    float *lq;            // LDS: 12 x 64W floats per slot parity, 16-byte aligned
+    static constexpr int kLq = 12;
 
     __device__ __forceinline__ void load(int j, uint32_t yoff, float (&w)[5]) const {
         const MemPriority prio;
@@ -199,30 +199,41 @@ struct LineCtx {
     // sweep quantities, then computes the fluxes at both of its interfaces
     // (the same HLL expression, operands in the same order, as the
     // neighbour that shares the interface), so no flux exchange and no second
-    // barrier is needed. Each lane's 12 quantities are consecutive with one
-    // float of padding: every LDS access is a lane base plus an immediate
-    // offset, and the odd stride keeps the 64 lanes on distinct banks. Two
-    // slots alternate by column parity, so one barrier per column suffices.
+    // barrier is needed. Each lane's 12 quantities are three consecutive
+    // 16-byte vectors: 3 vector writes and 6 vector reads per lane and column
+    // (measured 0.7% faster on 1024^3 and 3-5% on 64^3-128^3 than 12 scalar
+    // accesses per neighbour on a 13-float stride); the left neighbour's
+    // vectors are read and used before the right one's, which keeps the
+    // kernels within their register budgets. Two slots alternate by column
+    // parity, so one barrier per column suffices.
     __device__ __forceinline__ void zsweep(const Q &q, float k, float (&out)[5], int s) const {
         float *b = lq + (s & 1) * kLq * lanes;
-        float *own = b + gl * kLq;
-#pragma unroll
-        for (int i = 0; i < 5; ++i) {
-            own[i] = q.U[i];
-            own[5 + i] = q.F[i];
-        }
-        own[10] = q.up;
-        own[11] = q.um;
+        float4 *own = reinterpret_cast<float4 *>(b + gl * kLq);
+        own[0] = make_float4(q.U[0], q.U[1], q.U[2], q.U[3]);
+        own[1] = make_float4(q.U[4], q.F[0], q.F[1], q.F[2]);
+        own[2] = make_float4(q.F[3], q.F[4], q.up, q.um);
         __syncthreads();
-        const float *L = b + lidx * kLq;
-        const float *R = b + ridx * kLq;
-        const HllCoef cl = hll_coef(fmax3(L[10], q.up, 0.0f), fmin3(L[11], q.um, 0.0f));
-        const HllCoef cr = hll_coef(fmax3(q.up, R[10], 0.0f), fmin3(q.um, R[11], 0.0f));
+        const float4 *L4 = reinterpret_cast<const float4 *>(b + lidx * kLq);
+        const float4 *R4 = reinterpret_cast<const float4 *>(b + ridx * kLq);
+        float fl[5];
+        {
+            const float4 la = L4[0], lb = L4[1], lc = L4[2];
+            const float Lu[5] = {la.x, la.y, la.z, la.w, lb.x};
+            const float Lf[5] = {lb.y, lb.z, lb.w, lc.x, lc.y};
+            const HllCoef cl = hll_coef(fmax3(lc.z, q.up, 0.0f), fmin3(lc.w, q.um, 0.0f));
 #pragma unroll
-        for (int i = 0; i < 5; ++i) {
-            const float fl = hll_comp(cl, q.U[i] - L[i], cl.a * L[5 + i], q.F[i]);
-            const float fr = hll_comp(cr, R[i] - q.U[i], cr.a * q.F[i], R[5 + i]);
-            out[i] = fmaf(k, fl - fr, q.U[i]);
+            for (int i = 0; i < 5; ++i) fl[i] = hll_comp(cl, q.U[i] - Lu[i], cl.a * Lf[i], q.F[i]);
+        }
+        {
+            const float4 ra = R4[0], rb = R4[1], rc = R4[2];
+            const float Ru[5] = {ra.x, ra.y, ra.z, ra.w, rb.x};
+            const float Rf[5] = {rb.y, rb.z, rb.w, rc.x, rc.y};
+            const HllCoef cr = hll_coef(fmax3(q.up, rc.z, 0.0f), fmin3(q.um, rc.w, 0.0f));
+#pragma unroll
+            for (int i = 0; i < 5; ++i) {
+                const float fr = hll_comp(cr, Ru[i] - q.U[i], cr.a * q.F[i], Rf[i]);
+                out[i] = fmaf(k, fl[i] - fr, q.U[i]);
+            }
         }
     }
     __device__ __forceinline__ bool out_ok(int s) const { return valid && xs0 + s < nx; }
@@ -631,7 +642,7 @@ __device__ __forceinline__ void line_block(const SimDev *__restrict__ sims, cons
 template <int TX, int W>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(line_waves_per_simd<TX>()))) void k_line_sims(
     const SimDev *__restrict__ sims, const int *__restrict__ prefix, int nsims) {
-    __shared__ float lq[2 * 13 * 64 * W];
+    __shared__ __attribute__((aligned(16))) float lq[2 * LineCtx<TX>::kLq * 64 * W];
     line_block<TX>(sims, prefix, nsims, W, lq);
 }
 
@@ -641,7 +652,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(line_wav
 template <int TX, int MAXW>
 __global__ __launch_bounds__(64 * MAXW) __attribute__((amdgpu_waves_per_eu(line_waves_per_simd<TX>()))) void k_line_wide(
     const SimDev *__restrict__ sims, const int *__restrict__ prefix, int nsims, int W) {
-    __shared__ float lq[2 * 13 * 64 * MAXW];
+    __shared__ __attribute__((aligned(16))) float lq[2 * LineCtx<TX>::kLq * 64 * MAXW];
     line_block<TX>(sims, prefix, nsims, W, lq);
 }
 
