@@ -25,10 +25,14 @@ Pinning (every function has at least one independent check in tests/):
       periodic conservation, uniform-velocity advection keeps v and p exact,
       mirror symmetry, convergence to the exact (Toro) Sod solution on every
       axis, scalar C restatement bit-equal in float64.
-  _open_boundaries      equals an index clamp (independent formulation).
+  _open_boundaries      equals an index clamp for one ghost cell and a
+                        separable index map for 0-3 ghost cells down to
+                        n = g + 1 (independent formulations, brute force).
   _cfl_time_step        closed form for uniform states; per-cell identity.
-  time_integration      iteration counts 24/51/79/108 to t=0.2 held in the
-                        reference's results/scaling_data*.npz.
+  time_integration      iteration counts 24/51/79 to t=0.2 held in the
+                        reference's results/scaling_data*.npz; the
+                        reference's own check figures (figures/check_{N}.png)
+                        rendered from its runs.
 """
 
 from __future__ import annotations
