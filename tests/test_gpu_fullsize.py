@@ -27,10 +27,10 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 TOL = 1e-5
 
 
-def run_resident(buf0, dx, gamma, steps, tile_x=0):
+def run_resident(buf0, dx, gamma, steps, tile_x=0, handle_boundaries=False):
     from tinyfluids_amd.batch import Batch, Simulation
     buf1 = torch.empty_like(buf0)
-    sim = Simulation(buf0, dx, gamma, max_steps=steps, buffers=(buf0, buf1))
+    sim = Simulation(buf0, dx, gamma, max_steps=steps, buffers=(buf0, buf1), handle_boundaries=handle_boundaries)
     b = Batch([sim], tile_x=tile_x)
     sim.reset()
     b.step(steps)
@@ -73,6 +73,32 @@ def test_kelvin_helmholtz_full_size(shape, strict):
     assert abs(st.t - t_ref) <= 1e-5 * t_ref
     assert abs(tot[0] - tot0[0]) <= 1e-7 * tot0[0]
     assert abs(tot[4] - tot0[4]) <= 1e-5 * tot0[4]
+
+
+def test_kelvin_helmholtz_1024_open_boundaries():
+    """Open boundaries at 2^30 cells, where the clamped loads reach the last
+    x-plane, y-row and z-cell of 4 GiB fields: KH 1024^3, 20 steps with
+    handle_boundaries, against the float64 oracle on the 1024 x 1024 x 2
+    grid. The state is z-uniform and the z-clamp keeps it so, so that grid
+    is the exact reduction of the 3-D oracle."""
+    n, steps = 1024, 20
+    dx = 1.0 / n
+    print(f"KH {n}^3 open boundaries, {steps} steps", flush=True)
+    plane = ics.kelvin_helmholtz_plane(n, n, dx)
+    buf0 = torch.from_numpy(plane).to(DEV)[:, :, :, None].expand(5, n, n, n).contiguous()
+    out, st = run_resident(buf0, dx, ics.GAMMA_KH, steps, handle_boundaries=True)
+    assert torch.equal(out, out[:, :, :, :1].expand_as(out))
+    assert int(torch.count_nonzero(out[3])) == 0
+    got = out[:, :, :, :1].cpu().numpy()
+    del out, buf0
+    torch.cuda.empty_cache()
+    ref, t_ref, _ = c_oracle.run_steps(np.repeat(plane[:, :, :, None], 2, axis=3).astype(np.float64), dx,
+                                       ics.GAMMA_KH, steps, handle_boundaries=True)
+    err = npo.relative_l1(got, ref[:, :, :, :1], ics.GAMMA_KH)
+    print("KH open boundaries rel L1 per conserved field", err, "t", st.t, t_ref, flush=True)
+    assert np.all(err[[0, 1, 2, 4]] < TOL), err
+    assert err[3] == 0.0
+    assert abs(st.t - t_ref) <= 1e-5 * t_ref
 
 
 def test_blast_noise_512_against_float64_fixture():
