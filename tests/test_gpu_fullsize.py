@@ -6,6 +6,8 @@ round trip, so the float64 oracle on the nx x ny x 1 grid is the exact
 reduction of the 3-D oracle (bit-for-bit in the oracle). The GPU result must
 have identical z-planes, match the reduced oracle to relative L1 <= 1e-5 per
 conserved field, keep m_z exactly zero and conserve mass in float64 totals.
+The same at 1024^3 with open boundaries, against the oracle on the
+1024 x 1024 x 2 grid (the z-clamp keeps the state z-uniform).
 
 512^3 blast with seeded density noise (gamma 5/3, 200 steps): compared with
 the float64 oracle fixture tests/golden/blast512_noise.npz (every 32nd cell
