@@ -169,28 +169,38 @@ def measured_traffic(nx, ny, nz, kernel, tile):
     """HBM bytes per launch of the fused step from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary (scaled by a plain copy's counter reading),
     when it was taken on this workload, kernel and tiling; else None."""
-    path = os.path.join(ROOT, "profiles", "fused_step_kh1024_hbm_traffic.md")
-    if (nx, ny, nz) != (1024, 1024, 1024) or not os.path.exists(path):
+    if (nx, ny, nz) != (1024, 1024, 1024):
         return None, None
-    fields = {}
-    for line in open(path):
-        if ":" in line and line.split(":")[0] in ("traffic_bytes_per_launch", "tile", "kernel"):
-            k, v = line.split(":", 1)
-            fields[k.strip()] = v.strip()
-    if fields.get("tile") != f"{tile[0]},{tile[1]}" or fields.get("kernel") != kernel:
-        return None, None
-    return int(fields["traffic_bytes_per_launch"]), os.path.relpath(path, ROOT)
+    for path in profile_paths("fused_step_kh1024_hbm_traffic"):
+        fields = {}
+        for line in open(path):
+            if ":" in line and line.split(":")[0] in ("traffic_bytes_per_launch", "tile", "kernel"):
+                k, v = line.split(":", 1)
+                fields[k.strip()] = v.strip()
+        if fields.get("tile") == f"{tile[0]},{tile[1]}" and fields.get("kernel") == kernel:
+            return int(fields["traffic_bytes_per_launch"]), os.path.relpath(path, ROOT)
+    return None, None
+
+
+def profile_paths(stem):
+    """Committed summaries of one kind: profiles/<stem>.md (the default
+    tiling) and profiles/<stem>_tile*.md (other tilings the tuner picks)."""
+    import glob
+    return sorted(glob.glob(os.path.join(ROOT, "profiles", stem + ".md"))) + \
+        sorted(glob.glob(os.path.join(ROOT, "profiles", stem + "_tile*.md")))
 
 
 def measured_valu(nx, ny, nz, kernel, tile):
     """VALU instructions per cell-update and the VALU-busy fraction of the
     fused step from the committed rocprofv3 SQ-counter summary, when it was
     taken on this workload, kernel and tiling; else None."""
-    path = os.path.join(ROOT, "profiles", "fused_step_kh1024_valu.md")
-    if (nx, ny, nz) != (1024, 1024, 1024) or not os.path.exists(path):
+    if (nx, ny, nz) != (1024, 1024, 1024):
         return None
-    text = open(path).read()
-    if f"`{kernel}`" not in text or f"(tile_x,tile_y = {tile[0]},{tile[1]})" not in text:
+    for path in profile_paths("fused_step_kh1024_valu"):
+        text = open(path).read()
+        if f"`{kernel}`" in text and f"(tile_x,tile_y = {tile[0]},{tile[1]})" in text:
+            break
+    else:
         return None
     out = {"source": os.path.relpath(path, ROOT)}
     for line in text.splitlines():
