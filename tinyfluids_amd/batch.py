@@ -68,6 +68,9 @@ class Simulation:
         _lib.check(_lib.load().tf_sim_reset(ctypes.byref(self.desc), _stream(self.buf[0].device)))
 
     def status(self) -> _lib.SimStatus:
+        """t, dt, step count and flags. Reads the control block on the current
+        stream: work that advances the simulation on another stream must be
+        waited for first (Batch.status reads on the batch's stream)."""
         raw = self.ctrl[:48].cpu().numpy().tobytes()
         return _lib.SimStatus.from_buffer_copy(raw)
 
