@@ -81,15 +81,28 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t field_rsrc_indexed(const float
 // lane-masked block. Emits no instruction.
 __device__ __forceinline__ void pin_vgpr(float &v) { asm volatile("" : "+v"(v)); }
 
-// Raised wave priority while a wave issues a column's loads or stores. The
-// SIMD's arbiter then issues them ahead of the other resident waves'
-// arithmetic, so more requests are in flight. Measured: 4.4% faster on the
-// 1024^3 step, 2-3% on 256- and 512-cell lines, neutral on 62-cell tiles;
-// levels 1-3 alike, loads and stores together best.
+// Wave priority while a wave issues memory instructions: the SIMD's arbiter
+// issues a higher-priority wave's instructions ahead of the other resident
+// waves' arithmetic. Stores go at 2 (MemPriority). In the whole-line tiles
+// the loads of the two x-halo columns (the neighbouring tiles' columns,
+// mostly L2 hits) go at 1 and those of the tile's own columns at 0, each
+// column's loads between two s_setprio, which also keeps the compiler from
+// moving them across (line_load_priority); the 62-cell tiles load at 2.
+// Measured on the 1024^3 step: 9.56 ms without priorities, 9.08 ms with
+// every load and store at 2, 8.79 ms with this order (own loads at or above
+// the stores' level, or without the s_setprio around them, were slower);
+// 256^3 and 1536x512x512 gain another 1-2%, the 62-cell tiles nothing.
 struct MemPriority {
     __device__ __forceinline__ MemPriority() { __builtin_amdgcn_s_setprio(2); }
     __device__ __forceinline__ ~MemPriority() { __builtin_amdgcn_s_setprio(0); }
 };
+// Loaded column j of a whole-line tile of TX columns (0 and TX + 1 are its
+// x-halo): priority before (on) and after (!on) its loads.
+template <int TX>
+__device__ __forceinline__ void line_load_priority(int j, bool on) {
+    if (on && (j == 0 || j == TX + 1)) __builtin_amdgcn_s_setprio(1);
+    else __builtin_amdgcn_s_setprio(0);
+}
 
 __device__ __forceinline__ float sgpr_f(float v) {
     return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
@@ -191,9 +204,10 @@ struct LineCtx {
     static constexpr int kLq = 12;
 
     __device__ __forceinline__ void load(int j, uint32_t yoff, float (&w)[5]) const {
-        const MemPriority prio;
+        line_load_priority<TX>(j, true);
 #pragma unroll
         for (int f = 0; f < 5; ++f) w[f] = tf_struct_load_f32(rin[f], (int)xi[j], 0, (int)yoff, 0);
+        line_load_priority<TX>(j, false);
     }
     // The Z sweep with one LDS exchange per column: every lane publishes its
     // sweep quantities, then computes the fluxes at both of its interfaces
