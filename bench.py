@@ -190,6 +190,23 @@ def profile_paths(stem):
         sorted(glob.glob(os.path.join(ROOT, "profiles", stem + "_tile*.md")))
 
 
+def measured_trace(nx, ny, nz, kernel, tile):
+    """Average launch duration of the fused step in the committed
+    rocprofv3 --kernel-trace --stats summary of this workload, kernel and
+    tiling (to compare with kernel_ms, measured here with HIP events); else
+    None."""
+    import re
+    if (nx, ny, nz) != (1024, 1024, 1024):
+        return None
+    for path in profile_paths("fused_step_kh1024_kernel_trace"):
+        text = open(path).read()
+        if f"`{kernel}`" in text and f"(tile_x,tile_y = {tile[0]},{tile[1]})" in text:
+            m = re.search(r"Average fused-step launch: \*\*([0-9.]+) ms\*\*", text)
+            if m:
+                return {"source": os.path.relpath(path, ROOT), "avg_ms": float(m.group(1))}
+    return None
+
+
 def measured_valu(nx, ny, nz, kernel, tile):
     """VALU instructions per cell-update and the VALU-busy fraction of the
     fused step from the committed rocprofv3 SQ-counter summary, when it was
@@ -385,6 +402,7 @@ def main():
                      "algorithmic_bytes_per_launch": BYTES_PER_CELL * cells,
                      "copy_ceiling": m["copy_gbs"],
                      "valu": measured_valu(nx, ny, nz, kernel, (tx, ly)),
+                     "kernel_trace": measured_trace(nx, ny, nz, kernel, (tx, ly)),
                      "frac_of_copy_ceiling": achieved / m["copy_gbs"] if m["copy_gbs"] else None},
         "sim_time": st.t,
     }

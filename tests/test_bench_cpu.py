@@ -37,6 +37,17 @@ def test_traffic_profiles_parse_for_their_tiling(path):
     assert valu is not None and 0.0 < valu["busy"] <= 1.0 and 150 < valu["insts_per_cell"] < 400
 
 
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(ROOT, "profiles", "fused_step_kh1024_kernel_trace*.md"))))
+def test_kernel_trace_profiles_parse_for_their_tiling(path):
+    text = open(path).read()
+    tile = tuple(int(v) for v in text.split("(tile_x,tile_y = ")[1].split(")")[0].split(","))
+    kernel = "tf::k_line_wide<2, 16>"
+    assert f"`{kernel}`" in text
+    tr = bench.measured_trace(1024, 1024, 1024, kernel, tile)
+    assert tr["source"] == os.path.relpath(path, ROOT)
+    assert 5.0 < tr["avg_ms"] < 12.0  # 1024^3 steps: 42.95 GB at 3.6-8.6 TB/s
+
+
 def test_profiles_of_another_tiling_or_grid_are_not_reported():
     assert bench.measured_traffic(1024, 1024, 1024, "tf::k_line_wide<2, 16>", (2, 7)) == (None, None)
     assert bench.measured_traffic(512, 512, 512, "tf::k_line_wide<2, 16>", (2, 128)) == (None, None)
