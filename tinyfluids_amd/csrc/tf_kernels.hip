@@ -19,6 +19,9 @@
 //   - Epilogue: each output cell contributes max_d |v_d| + c to the next CFL
 //     time step (fluid.py:425-504) through an integer max on float bits,
 //     which also propagates NaN, so the next dt needs no second read.
+//   - Memory issue: raised wave priorities around stores and x-halo loads
+//     (MemPriority, line_load_priority) let a wave's memory instructions
+//     pass the other resident waves' arithmetic.
 //
 // Algorithmic traffic is 40 bytes per cell-update (5 float32 read, 5 written);
 // the halo rows and columns are re-read from L2.
