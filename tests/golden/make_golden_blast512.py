@@ -7,7 +7,13 @@ The full state is 5.4 GB per float64 copy, so the fixture keeps
 - every 32nd cell along each axis (16^3 cells x 5 primitive fields) at steps
   0, 50, 100, 150 and 200 (float64),
 - the float64 totals of mass, momentum and energy every 10 steps,
-- the float64 dt of every step.
+- the float64 dt of every step;
+and tests/golden/blast512_planes.npz the full-resolution mid-planes through
+the blast centre, one per axis (x = 256, y = 256, z = 256: 512 x 512 cells x
+5 primitive fields each), at steps 100 and 200. The planes are stored as
+float32 roundings of the float64 oracle (relative rounding <= 6e-8 per value,
+far below the 1e-5 relative L1 bound they are compared at) to keep the file
+small.
 
 Runs the scalar C oracle (OpenMP) for about half an hour on 8 cores.
 Run from the repository root: python tests/golden/make_golden_blast512.py
@@ -32,6 +38,8 @@ N = 512
 STEPS = 200
 SUB = 32
 CHECK = (0, 50, 100, 150, 200)
+PLANE_STEPS = (100, 200)
+MID = N // 2
 
 
 def main():
@@ -40,6 +48,7 @@ def main():
     del s32
     g = ics.GAMMA_BLAST
     sub = {}
+    planes = {}
     totals = []
     dts = []
     t0 = time.time()
@@ -48,6 +57,10 @@ def main():
             totals.append(c_oracle.conserved_totals(state, g))
         if step in CHECK:
             sub[f"sub_{step}"] = state[:, ::SUB, ::SUB, ::SUB].copy()
+        if step in PLANE_STEPS:
+            planes[f"x_{step}"] = state[:, MID, :, :].astype(np.float32)
+            planes[f"y_{step}"] = state[:, :, MID, :].astype(np.float32)
+            planes[f"z_{step}"] = state[:, :, :, MID].astype(np.float32)
         if step == STEPS:
             break
         _, n, _, d = c_oracle.time_integration(state, dx, float("inf"), g, False, 10, 0.4, in_place=True,
@@ -58,6 +71,8 @@ def main():
     np.savez_compressed(os.path.join(HERE, "blast512_noise.npz"), totals=np.array(totals), dts=np.array(dts),
                         stride=np.array(SUB), checkpoints=np.array(CHECK), dx=np.array(dx), gamma=np.array(g),
                         **sub)
+    np.savez_compressed(os.path.join(HERE, "blast512_planes.npz"), index=np.array(MID),
+                        steps=np.array(PLANE_STEPS), **planes)
     print("done", time.time() - t0)
 
 

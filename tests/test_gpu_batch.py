@@ -117,6 +117,8 @@ def test_batched_meets_oracle():
 
 
 def test_server_results_same_alone_and_in_mix():
+    """Served jobs (three at once, then a trickle) equal the same jobs run
+    alone, bit for bit, and the float64 oracle within 1e-5."""
     from tinyfluids_amd import engine
     specs = specs_mixed()
     for k, s in enumerate(specs):
@@ -129,6 +131,12 @@ def test_server_results_same_alone_and_in_mix():
         assert a.steps_done == a.spec.steps
         assert torch.equal(a.state, b.state), a.index
         assert a.finished >= a.arrival
+        # and each served result meets the float64 oracle (5-11 steps)
+        ref, t_ref, _ = c_oracle.run_steps(a.spec.initial_state().astype(np.float64), a.spec.dx, a.spec.gamma,
+                                           a.spec.steps)
+        err = npo.relative_l1(a.state.cpu().numpy(), ref, a.spec.gamma)
+        assert np.all(err < 1e-5), (a.index, err)
+        assert abs(a.t - t_ref) < 1e-6 * t_ref, (a.index, a.t, t_ref)
 
 
 def test_server_respects_hbm_budget():

@@ -75,6 +75,24 @@ def test_conversions(gamma):
     np.testing.assert_allclose(back.cpu().numpy(), s, rtol=1e-5, atol=1e-6)
 
 
+@pytest.mark.parametrize("gamma", [5 / 3, 1.4])
+def test_primitive_from_conserved_against_oracle(gamma):
+    """primitive_state_from_conserved (fluid.py:69-103) on one conserved
+    input, against the float64 oracle on the same float32 values."""
+    rng = np.random.default_rng(11)
+    prim = random_state((7, 6, 65), 9)
+    prim[1:4] *= 2.0  # kinetic energy up to about 60% of E
+    cons = npo.conserved_state_from_primitive(prim.astype(np.float64), gamma).astype(np.float32)
+    cons[1:4, 0, 0, :3] = 0.0  # zero momentum: v must be exactly 0
+    cons[0] *= rng.uniform(1.0, 2.0, cons[0].shape).astype(np.float32)  # other rho, p stays > 0
+    got = tf().primitive_state_from_conserved(to_dev(cons), gamma).cpu().numpy()
+    ref = npo.primitive_state_from_conserved(cons.astype(np.float64), gamma)
+    np.testing.assert_allclose(got[[0, 1, 2, 3]], ref[[0, 1, 2, 3]], rtol=2e-6, atol=0)
+    assert np.all(got[1:4, 0, 0, :3] == 0.0)
+    # p = (gamma-1)(E - m.v/2) loses bits to the subtraction: bound by E's scale
+    np.testing.assert_allclose(got[4], ref[4], rtol=0, atol=4e-7 * np.abs(cons[4]).max())
+
+
 @pytest.mark.parametrize("shape", [(8, 8, 8), (5, 9, 70), (33, 3, 130)])
 def test_cfl_time_step(shape):
     s = random_state(shape, 2)

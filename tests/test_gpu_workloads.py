@@ -1,0 +1,107 @@
+"""GPU tests at the batched and served workloads, against the float64 oracle.
+
+Batched: the seeded uneven batch of 256 blasts (sides {32,48,64,96,128}^3,
+centre U[0.2,0.8]^3, radius U[0.05,0.15], pressure ratio log-uniform 2-100,
+gamma in {1.4, 5/3}, 100 steps each; ics.uneven_batch(256, 100, seed=3)),
+advanced together by captured-graph replays:
+  - every simulation bit-equal to the plain single-grid loop (run_steps);
+  - one simulation of each side length against the float64 oracle: relative
+    L1 <= 1e-5 per conserved field and t within 1e-6.
+
+Served: 12 jobs of the seeded Poisson stream ics.serving_stream(12, 400,
+seed=4) (grids up to 256 per side, aspect ratios up to 4:1, 65-1742 steps,
+blasts and shear layers, all arriving within 45 ms, so they overlap):
+  - every served result bit-equal to the same job run alone (serve_plain);
+  - every shear-layer job against the float64 oracle on its exact
+    nx x ny x 1 reduction (the shear layer does not depend on z and the
+    z-sweep of a z-uniform state is a round trip), and every blast job of at
+    most 1e8 cell-updates against the full float64 oracle.
+    Runs of hundreds to thousands of steps accumulate float32 round-off, and
+    the reference algorithm run in float32 (its own precision) is itself up
+    to 8.7e-4 away from float64 on the small m_y field of a 1742-step shear
+    layer. The bound per conserved field is therefore max(1e-5, 2 x the
+    relative L1 of the float32 reference against float64), computed here by
+    the oracle in both precisions; fields that are zero in the oracle must be
+    exactly zero, and t must match within max(1e-6, 2 x the float32
+    reference's relative t error).
+
+The reference's loop semantics these follow: time_integration.py:63-92.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import c_oracle, np_tinyfluids as npo
+from tinyfluids_amd import ics
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda:0")
+TOL = 1e-5
+
+
+def oracle_for(spec):
+    """(initial state of the oracle run, index that maps the GPU state onto
+    it): the nx x ny x 1 reduction for shear-layer jobs, else the full grid."""
+    if spec.kind == "kh":
+        nx, ny, _ = spec.shape
+        return ics.kelvin_helmholtz_plane(nx, ny, spec.dx)[:, :, :, None], (slice(None),) * 3 + (slice(0, 1),)
+    return spec.initial_state(), (slice(None),) * 4
+
+
+def check_against_oracle(spec, got, t, relaxed):
+    """got: the GPU state (torch), t: its simulated time."""
+    s0, view = oracle_for(spec)
+    if spec.kind == "kh":
+        assert torch.equal(got, got[:, :, :, :1].expand_as(got)), "shear layer must stay z-uniform"
+    ref, t_ref, _ = c_oracle.run_steps(s0.astype(np.float64), spec.dx, spec.gamma, spec.steps)
+    err = npo.relative_l1(got[view].cpu().numpy(), ref, spec.gamma)
+    bound, tbound = np.full(5, TOL), 1e-6
+    if relaxed:
+        ref32, t32, _ = c_oracle.run_steps(s0.astype(np.float32), spec.dx, spec.gamma, spec.steps)
+        err32 = npo.relative_l1(ref32, ref, spec.gamma)
+        bound = np.maximum(TOL, 2.0 * err32)
+        tbound = max(1e-6, 2.0 * abs(t32 - t_ref) / t_ref)
+    print(f"{spec.kind} {spec.shape} {spec.steps} steps: rel L1 {err} bound {bound}, t {t} vs {t_ref}", flush=True)
+    zero = np.abs(c_oracle.prim_to_cons(ref, spec.gamma)).reshape(5, -1).sum(axis=1) == 0
+    assert np.all(err[zero] == 0.0), err
+    assert np.all(err[~zero] <= bound[~zero]), (spec.shape, spec.steps, err, bound)
+    assert abs(t - t_ref) <= tbound * t_ref, (t, t_ref)
+
+
+def test_uneven_batch_256_bitwise_and_against_oracle():
+    from tinyfluids_amd import engine
+    specs = ics.uneven_batch(256, 100, seed=3)
+    print("uneven batch: 256 blasts, sides {32,48,64,96,128}^3, 100 steps", flush=True)
+    init = engine.upload(specs, DEV)
+    b = engine.run_batched(specs, init, graph=True)
+    p = engine.run_plain(specs, init)
+    for i, (x, y, st) in enumerate(zip(b.states, p.states, b.statuses)):
+        assert st.n_iter == 100 and not st.active and not st.nonfinite, i
+        assert torch.equal(x, y), f"sim {i} {specs[i].shape}"
+    first = {}
+    for i, s in enumerate(specs):
+        first.setdefault(s.shape[0], i)
+    assert sorted(first) == list(ics.BATCH_SIDES)
+    for side, i in sorted(first.items()):
+        check_against_oracle(specs[i], b.states[i], b.statuses[i].t, relaxed=False)
+
+
+def test_served_stream_bitwise_and_against_oracle():
+    from tinyfluids_amd import engine
+    specs = ics.serving_stream(12, 400.0, seed=4)
+    print("served stream: 12 jobs, Poisson rate 400/s", flush=True)
+    assert max(max(s.shape) / min(s.shape) for s in specs) == 4
+    srv = engine.Server(hbm_budget_bytes=32 << 30, keep_states=True)
+    served = srv.serve(specs)
+    alone = engine.serve_plain(specs, keep_states=True)
+    assert served.max_resident >= 3  # the jobs really shared the device
+    checked = {"kh": 0, "blast": 0}
+    for a, b in zip(served.jobs, alone.jobs):
+        assert a.steps_done == a.spec.steps and not a.nonfinite, a.index
+        assert a.t == b.t, a.index
+        assert torch.equal(a.state, b.state), f"job {a.index} {a.spec.shape}"
+        if a.spec.kind == "kh" or a.spec.cells * a.spec.steps <= 1e8:
+            check_against_oracle(a.spec, a.state, a.t, relaxed=True)
+            checked[a.spec.kind] += 1
+    assert checked["kh"] >= 6 and checked["blast"] >= 3, checked

@@ -17,6 +17,7 @@
 
 #include "../../include/tinyfluids.h"
 #include "tf_internal.h"
+#include "tf_plan.h"
 
 namespace tf {
 hipError_t launch_step_direct(int tx, const SimDev &S, const float *in, float *out, const double *dt, double dx,
@@ -95,12 +96,19 @@ struct EventPair {
     }
 };
 
-int check_dims(int nx, int ny, int nz) {
-    TF_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1, "grid dimensions must be >= 1 (got %d x %d x %d)", nx, ny, nz);
-    const double cells = (double)nx * ny * nz;
-    TF_REQUIRE(cells * 5.0 < 9.0e18, "grid too large");
-    TF_REQUIRE((double)nx * ny < 2.0e9 && (double)ny * nz < 2.0e9, "grid planes exceed 2^31 cells");
-    return TF_OK;
+// Argument checks of tf_plan.cpp, with the message kept for tf_last_error.
+int req(int rc, const std::string &msg) { return rc == TF_OK ? TF_OK : fail(rc, "%s", msg.c_str()); }
+int req_dims(int nx, int ny, int nz) {
+    std::string e;
+    return req(check_dims(nx, ny, nz, e), e);
+}
+int req_fused_size(int nx, int ny, int nz) {
+    std::string e;
+    return req(check_fused_size(nx, ny, nz, e), e);
+}
+int req_open_bc(int open_bc, int nx, int ny, int nz) {
+    std::string e;
+    return req(check_open_bc(open_bc, nx, ny, nz, e), e);
 }
 
 Gas make_gas(double gamma, float k = 0.0f) {
@@ -113,25 +121,13 @@ Gas make_gas(double gamma, float k = 0.0f) {
 }
 
 // ------------------------------------------------------------- tiling registry
+// Tilings measured or set per grid shape:
+// - g_tiling: the single explicit-dt step (tf_autotune_evolve, tf_set_tiling);
+// - g_loop_tiling: the plan that loops and batches run (tf_autotune_loop,
+//   tf_set_tiling), tile_y 0 letting the plan choose.
 std::mutex g_tiling_mu;
 std::map<std::tuple<int, int, int>, std::pair<int, int>> g_tiling;
 
-int default_tile_x(int nx) { return nx >= 8 ? 8 : (nx >= 4 ? 4 : 2); }
-
-// Rows per y-chunk: enough waves to fill the chip (256 CUs, a few waves per
-// SIMD) with several rounds so the last round is short, without letting the
-// two halo rows per chunk cost much.
-int default_tile_y(int nx, int ny, int nz, int tx) {
-    const long ntx = (nx + tx - 1) / tx, ntz = (nz + kZT - 1) / kZT;
-    const long per_chunk = ntx * ntz;
-    const long target = 256L * 4 * 3 * 6;  // ~6 rounds of full occupancy
-    int ly = ny;
-    while (ly > 32 && per_chunk * ((ny + ly - 1) / ly) < target) ly = (ly + 1) / 2;
-    return std::max(1, ly);
-}
-
-// Tiling of the one-simulation plan that tf_time_integration runs (set by
-// tf_autotune_loop or tf_set_tiling); tile_y 0 lets the plan choose.
 struct LoopTiling {
     int tx, ly;
     int line;  // 1: whole z-lines where the shape allows them; 0: 62-cell z-tiles
@@ -139,15 +135,6 @@ struct LoopTiling {
 std::map<std::tuple<int, int, int>, LoopTiling> g_loop_tiling;
 // Set by the loop tuner while it builds a candidate plan: -1 none, else 0/1.
 thread_local int tl_line_force = -1;
-
-// Untuned loop tiling: narrow tiles for grids too small to fill the chip,
-// so that each wave's serial stream stays short.
-int default_loop_tx(int nx, int ny, int nz) {
-    const double cells = (double)nx * ny * nz;
-    if (cells >= 16777216.0) return default_tile_x(nx);
-    if (cells >= 2097152.0) return nx >= 4 ? 4 : 2;
-    return 2;
-}
 
 void loop_tiling_for(int nx, int ny, int nz, int &tx, int &ly) {
     {
@@ -169,6 +156,8 @@ void loop_tiling_for(int nx, int ny, int nz, int &tx, int &ly) {
     ly = 0;
 }
 
+// The tiling of tf_evolve_state with 62-cell z-tiles: the single-step
+// setting of the shape (tf_autotune_evolve, tf_set_tiling), else defaults.
 void tiling_for(int nx, int ny, int nz, int &tx, int &ly) {
     {
         std::lock_guard<std::mutex> lk(g_tiling_mu);
@@ -183,103 +172,23 @@ void tiling_for(int nx, int ny, int nz, int &tx, int &ly) {
     ly = default_tile_y(nx, ny, nz, tx);
 }
 
-SimDev make_simdev(int nx, int ny, int nz, int open_bc, double gamma, int tx, int ly) {
-    SimDev S;
-    std::memset(&S, 0, sizeof S);
-    S.nx = nx;
-    S.ny = ny;
-    S.nz = nz;
-    S.open_bc = open_bc ? 1 : 0;
-    Gas g = make_gas(gamma);
-    S.gamma = g.gamma;
-    S.gm1 = g.gm1;
-    S.inv_gm1 = g.inv_gm1;
-    S.ly = std::max(1, std::min(ly, ny));
-    S.ntx = (nx + tx - 1) / tx;
-    S.ntz = (nz + kZT - 1) / kZT;
-    S.nyc = (ny + S.ly - 1) / S.ly;
-    S.waves = S.ntx * S.ntz * S.nyc;
-    // fields of exactly 4 GiB: the last dword is out of reach of a buffer resource
-    S.far = ((int64_t)nx * ny * nz == (int64_t)1 << 30) ? 1 : 0;
-    return S;
-}
-
-// The tiles of S that touch the last (x, y) row (the device-side predicate
-// touches_last_row, evaluated for every tile); appended for simulation `sim`.
-// With `split`, each tile's rows are cut into up to kFarSplit ranges (of at
-// least 4 rows): the far kernel runs after the main one, so its waves should
-// be many and short.
-void far_tiles(const SimDev &S, int tx, int sim, std::vector<FarTile> &out, bool split) {
-    if (!S.far || S.line_w > 0) return;  // line-tiled grids reach the last cell themselves
-    for (int yc = 0; yc < S.nyc; ++yc)
-        for (int xt = 0; xt < S.ntx; ++xt) {
-            const bool xr = xt == 0 || xt * tx + tx >= S.nx - 1;
-            const bool yr = yc == 0 || yc * S.ly + S.ly >= S.ny - 1;
-            if (!(xr && yr)) continue;
-            const int y0 = yc * S.ly, rows = std::min(S.ly, S.ny - y0);
-            const int sub = split ? std::max(4, (rows + kFarSplit - 1) / kFarSplit) : rows;
-            for (int r = 0; r < rows; r += sub) out.push_back(FarTile{sim, xt, y0 + r, std::min(sub, rows - r)});
-        }
-}
-
-FarList far_list(const SimDev &S, int tx) {
-    std::vector<FarTile> v;
-    far_tiles(S, tx, 0, v, false);
-    FarList F;
-    std::memset(&F, 0, sizeof F);
-    F.n = (int)std::min<size_t>(v.size(), kMaxFarPerSim);
-    for (int i = 0; i < F.n; ++i) F.t[i] = v[i];
-    return F;
-}
-
-bool valid_tx(int tx) { return tx == 2 || tx == 4 || tx == 6 || tx == 8; }
-
-// The fused stencil addresses each field with 32-bit byte offsets.
-int check_fused_size(int nx, int ny, int nz) {
-    if ((double)nx * ny * nz > 1073741824.0)
-        return fail(TF_ERR_UNSUPPORTED, "fused step: %d x %d x %d exceeds 2^30 cells per field", nx, ny, nz);
-    return TF_OK;
-}
-
-int check_open_bc(int open_bc, int nx, int ny, int nz) {
-    if (open_bc) TF_REQUIRE(nx >= 2 && ny >= 2 && nz >= 2, "open boundaries need every dimension >= 2");
-    return TF_OK;
-}
-
-constexpr int kMaxLineW = 16;          // whole z-lines of up to 1024 cells per workgroup
-constexpr int kClasses = kMaxLineW + 1;  // halo tiling + line tiling with W = 1..16
-std::atomic<int> g_line_mode{1};
-
-// Line tiling of a z-line of nz cells: w waves per workgroup holding
-// `groups` whole lines (x-column groups), lane after lane. The fewest waves
-// that hold a line, unless a few more (up to 8) hold several lines with at
-// least 2% fewer idle lanes: 48-cell lines go four to three waves, 96-cell
-// lines two to three waves (measured 6% faster on 48^3 and 96^3).
-void line_shape(int nz, int &w, int &groups) {
-    const int w0 = (nz + 63) / 64;
-    w = w0;
-    groups = std::max(1, 64 * w0 / nz);
-    double best = (double)(groups * nz) / (64.0 * w0);
-    for (int cw = w0 + 1; cw <= std::min(kMaxLineW, std::max(w0, 8)); ++cw) {
-        const int g = 64 * cw / nz;
-        const double use = (double)(g * nz) / (64.0 * cw);
-        if (use > best + 0.02) {
-            best = use;
-            w = cw;
-            groups = g;
+// ... and with whole z-lines: the single-step setting, else the loop tuner's
+// measurement of the shape (the same kernels), else defaults.
+void line_tiling_for(int nx, int ny, int nz, int &tx, int &ly) {
+    {
+        std::lock_guard<std::mutex> lk(g_tiling_mu);
+        auto it = g_tiling.find({nx, ny, nz});
+        if (it != g_tiling.end()) {
+            tx = it->second.first;
+            ly = it->second.second;
+            return;
         }
     }
+    loop_tiling_for(nx, ny, nz, tx, ly);
+    if (ly <= 0) ly = default_tile_y(nx, ny, nz, tx);
 }
-double halo_lane_use(int nz) { return (double)nz / (64.0 * ((nz + kZT - 1) / kZT)); }
-// Tile width of a line class: at most 4 (registers for two waves per SIMD),
-// and 2 above 8 waves per workgroup (four waves per SIMD).
-int line_tx(int w, int tx) { return w > 8 ? 2 : std::min(tx, 4); }
 
-double line_lane_use(int nz) {
-    int w, g;
-    line_shape(nz, w, g);
-    return (double)(g * nz) / (64.0 * w);
-}
+std::atomic<int> g_line_mode{1};
 
 // Whole z-lines are possible where they fit a workgroup and use the lanes at
 // least as well as 62-cell z-tiles (every row segment then starts at its
@@ -335,7 +244,7 @@ int tf_primitive_state_from_conserved(const float *cons, float *prim, int64_t nc
 
 int tf_cfl_time_step(const float *prim, int nx, int ny, int nz, double grid_spacing, double gamma, double c_cfl,
                      double *dt_out, float *max_speed_out, void *workspace, void *stream) {
-    if (int e = check_dims(nx, ny, nz)) return e;
+    if (int e = req_dims(nx, ny, nz)) return e;
     TF_REQUIRE(prim && workspace, "null pointer");
     TF_REQUIRE(dt_out || max_speed_out, "nothing to write: dt_out and max_speed_out are both NULL");
     auto *H = reinterpret_cast<WsHeader *>(workspace);
@@ -347,7 +256,7 @@ int tf_cfl_time_step(const float *prim, int nx, int ny, int nz, double grid_spac
 }
 
 int tf_open_boundaries(const float *in, float *out, int nx, int ny, int nz, int num_ghost_cells, void *stream) {
-    if (int e = check_dims(nx, ny, nz)) return e;
+    if (int e = req_dims(nx, ny, nz)) return e;
     TF_REQUIRE(in && out, "null state pointer");
     TF_REQUIRE(in != out, "in and out must not alias");
     TF_REQUIRE(num_ghost_cells >= 0, "num_ghost_cells must be >= 0 (got %d)", num_ghost_cells);
@@ -359,7 +268,7 @@ int tf_open_boundaries(const float *in, float *out, int nx, int ny, int nz, int 
 
 int tf_evolve_state_along_axis(const float *in, float *out, int nx, int ny, int nz, double grid_spacing,
                                const double *dt, double gamma, int axis, void *stream) {
-    if (int e = check_dims(nx, ny, nz)) return e;
+    if (int e = req_dims(nx, ny, nz)) return e;
     TF_REQUIRE(in && out && dt, "null pointer");
     TF_REQUIRE(in != out, "in and out must not alias");
     TF_REQUIRE(axis >= 1 && axis <= 3, "axis must be 1 (x), 2 (y) or 3 (z), got %d", axis);
@@ -367,53 +276,66 @@ int tf_evolve_state_along_axis(const float *in, float *out, int nx, int ny, int 
     return TF_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// tf_evolve_state with whole z-lines: the table and the control block (step
+// 0, k from dt) are written into the workspace header on the stream by a
+// setup kernel, so the call stays asynchronous.
+int evolve_line(const float *in, float *out, int nx, int ny, int nz, double grid_spacing, const double *dt,
+                double gamma, int handle_boundaries, unsigned int *max_speed_out, WsHeader *H, int tx, int ly,
+                hipStream_t st) {
+    SimDev S = make_simdev(nx, ny, nz, handle_boundaries, gamma, tx, ly);
+    make_line(S, tx);
+    S.buf[0] = const_cast<float *>(in);
+    S.buf[1] = out;
+    S.ctrl = &H->ctrl;
+    S.skip_max = max_speed_out ? 0 : 1;
+    const int nblocks = S.ntx * S.nyc;
+    TF_HIP(launch_direct_setup(S, &H->sim, H->prefix, nblocks, dt, grid_spacing, st));
+    TF_HIP(launch_line_sims(line_tx(S.line_w, tx), S.line_w, &H->sim, H->prefix, 1, nblocks, st));
+    if (max_speed_out) TF_HIP(launch_direct_bits(&H->ctrl, max_speed_out, st));
+    return TF_OK;
+}
+
+// tf_evolve_state with 62-cell z-tiles (no workspace needed).
+int evolve_halo(const float *in, float *out, int nx, int ny, int nz, double grid_spacing, const double *dt,
+                double gamma, int handle_boundaries, unsigned int *max_speed_out, int tx, int ly, hipStream_t st) {
+    const SimDev S = make_simdev(nx, ny, nz, handle_boundaries, gamma, tx, ly);
+    FarList F;
+    std::string e;
+    if (int rc = far_list(S, tx, F, e)) return fail(rc, "%s", e.c_str());
+    TF_HIP(launch_step_direct(tx, S, in, out, dt, grid_spacing, max_speed_out, st));
+    TF_HIP(launch_step_direct_far(tx, S, F, in, out, dt, grid_spacing, max_speed_out, st));
+    return TF_OK;
+}
+}  // namespace
+
+extern "C" {
+
 int tf_evolve_state(const float *in, float *out, int nx, int ny, int nz, double grid_spacing, const double *dt,
                     double gamma, int handle_boundaries, unsigned int *max_speed_out, void *workspace,
                     void *stream) {
-    if (int e = check_dims(nx, ny, nz)) return e;
-    if (int e = check_fused_size(nx, ny, nz)) return e;
+    if (int e = req_dims(nx, ny, nz)) return e;
+    if (int e = req_fused_size(nx, ny, nz)) return e;
     TF_REQUIRE(in && out && dt, "null pointer");
     TF_REQUIRE(in != out, "in and out must not alias");
-    if (int e = check_open_bc(handle_boundaries, nx, ny, nz)) return e;
+    if (int e = req_open_bc(handle_boundaries, nx, ny, nz)) return e;
     int tx, ly;
-    tiling_for(nx, ny, nz, tx, ly);
     if (workspace && use_line_tiling(nx, ny, nz)) {
-        // Whole z-lines, as in loops and batches: the table and the control
-        // block (step 0, k from dt) are written on the stream by a setup
-        // kernel, so the call stays asynchronous.
-        auto *H = reinterpret_cast<WsHeader *>(workspace);
-        int w, groups;
-        line_shape(nz, w, groups);
-        const int txl = line_tx(w, tx);
-        int lx, ll;
-        loop_tiling_for(nx, ny, nz, lx, ll);
-        SimDev S = make_simdev(nx, ny, nz, handle_boundaries, gamma, txl, ll > 0 ? ll : ly);
-        S.buf[0] = const_cast<float *>(in);
-        S.buf[1] = out;
-        S.ctrl = &H->ctrl;
-        S.line_w = w;
-        S.groups = groups;
-        S.ntx = (nx + groups * txl - 1) / (groups * txl);
-        S.ntz = 1;
-        S.waves = S.ntx * S.nyc * w;
-        S.skip_max = max_speed_out ? 0 : 1;
-        const int nblocks = S.ntx * S.nyc;
-        hipStream_t st = S_(stream);
-        TF_HIP(launch_direct_setup(S, &H->sim, H->prefix, nblocks, dt, grid_spacing, st));
-        TF_HIP(launch_line_sims(txl, w, &H->sim, H->prefix, 1, nblocks, st));
-        if (max_speed_out) TF_HIP(launch_direct_bits(&H->ctrl, max_speed_out, st));
-        return TF_OK;
+        line_tiling_for(nx, ny, nz, tx, ly);
+        return evolve_line(in, out, nx, ny, nz, grid_spacing, dt, gamma, handle_boundaries, max_speed_out,
+                           reinterpret_cast<WsHeader *>(workspace), tx, ly, S_(stream));
     }
-    const SimDev S = make_simdev(nx, ny, nz, handle_boundaries, gamma, tx, ly);
-    TF_HIP(launch_step_direct(tx, S, in, out, dt, grid_spacing, max_speed_out, S_(stream)));
-    TF_HIP(launch_step_direct_far(tx, S, far_list(S, tx), in, out, dt, grid_spacing, max_speed_out, S_(stream)));
-    return TF_OK;
+    tiling_for(nx, ny, nz, tx, ly);
+    return evolve_halo(in, out, nx, ny, nz, grid_spacing, dt, gamma, handle_boundaries, max_speed_out, tx, ly,
+                       S_(stream));
 }
 
 
 int tf_conserved_totals(const float *prim, int nx, int ny, int nz, double gamma, double *out5, void *workspace,
                         void *stream) {
-    if (int e = check_dims(nx, ny, nz)) return e;
+    if (int e = req_dims(nx, ny, nz)) return e;
     TF_REQUIRE(prim && out5 && workspace, "null pointer");
     double *partials = reinterpret_cast<double *>(static_cast<char *>(workspace) + kWsHeaderBytes);
     TF_HIP(launch_totals(prim, (int64_t)nx * ny * nz, gamma, partials, out5, S_(stream)));
@@ -435,14 +357,14 @@ int tf_hbm_copy(const void *src, void *dst, size_t bytes, void *stream) {
 
 // ------------------------------------------------------------------ tiling
 int tf_get_tiling(int nx, int ny, int nz, int *tile_x, int *tile_y) {
-    if (int e = check_dims(nx, ny, nz)) return e;
+    if (int e = req_dims(nx, ny, nz)) return e;
     TF_REQUIRE(tile_x && tile_y, "null pointer");
     tiling_for(nx, ny, nz, *tile_x, *tile_y);
     return TF_OK;
 }
 
 int tf_set_tiling(int nx, int ny, int nz, int tile_x, int tile_y) {
-    if (int e = check_dims(nx, ny, nz)) return e;
+    if (int e = req_dims(nx, ny, nz)) return e;
     TF_REQUIRE(valid_tx(tile_x), "tile_x must be 2, 4, 6 or 8 (got %d)", tile_x);
     TF_REQUIRE(tile_y >= 1, "tile_y must be >= 1");
     std::lock_guard<std::mutex> lk(g_tiling_mu);
@@ -453,8 +375,8 @@ int tf_set_tiling(int nx, int ny, int nz, int tile_x, int tile_y) {
 
 int tf_autotune_evolve(const float *in, float *out, int nx, int ny, int nz, double grid_spacing, double gamma,
                        void *workspace, int *tile_x, int *tile_y, float *ms_per_step, void *stream) {
-    if (int e = check_dims(nx, ny, nz)) return e;
-    if (int e = check_fused_size(nx, ny, nz)) return e;
+    if (int e = req_dims(nx, ny, nz)) return e;
+    if (int e = req_fused_size(nx, ny, nz)) return e;
     TF_REQUIRE(in && out && workspace && tile_x && tile_y, "null pointer");
     TF_REQUIRE(in != out, "in and out must not alias");
     hipStream_t st = S_(stream);
@@ -467,26 +389,35 @@ int tf_autotune_evolve(const float *in, float *out, int nx, int ny, int nz, doub
     std::vector<int> lys;
     for (int v : {8, 16, 32, 64, 128, 256, 512}) if (v < ny) lys.push_back(v);
     lys.push_back(ny);
+    // The candidates of the kernels tf_evolve_state runs for this shape with
+    // a workspace: whole z-lines (tile width 2 or 4) or 62-cell z-tiles.
+    const bool line = use_line_tiling(nx, ny, nz);
+    const std::vector<int> txs = line ? std::vector<int>{2, 4} : std::vector<int>{2, 4, 6, 8};
     EventPair ev;
     TF_HIP(ev.create());
     hipEvent_t e0 = ev.e0, e1 = ev.e1;
     float best = 1e30f;
     int btx = 4, bly = ny;
     int rc = TF_OK;
-    for (int tx : {2, 4, 6, 8}) {
+    std::vector<std::pair<int, int>> seen;  // resolved (tile width, rows)
+    for (int tx : txs) {
         if (tx > 2 && tx > nx) continue;
         for (int ly : lys) {
-            const SimDev S = make_simdev(nx, ny, nz, 0, gamma, tx, ly);
-            const FarList F = far_list(S, tx);
-            hipError_t e = launch_step_direct(tx, S, in, out, dt, grid_spacing, nullptr, st);
-            if (e == hipSuccess) e = launch_step_direct_far(tx, S, F, in, out, dt, grid_spacing, nullptr, st);
+            SimDev S = make_simdev(nx, ny, nz, 0, gamma, tx, ly);
+            if (line) make_line(S, tx);
+            const std::pair<int, int> key{line ? line_tx(S.line_w, tx) : tx, S.ly};
+            if (std::find(seen.begin(), seen.end(), key) != seen.end()) continue;
+            seen.push_back(key);
+            auto step = [&]() {
+                return line ? evolve_line(in, out, nx, ny, nz, grid_spacing, dt, gamma, 0, nullptr, H, tx, ly, st)
+                            : evolve_halo(in, out, nx, ny, nz, grid_spacing, dt, gamma, 0, nullptr, tx, ly, st);
+            };
             const int reps = 3;
-            if (e == hipSuccess) e = hipEventRecord(e0, st);
-            for (int r = 0; r < reps && e == hipSuccess; ++r) {
-                e = launch_step_direct(tx, S, in, out, dt, grid_spacing, nullptr, st);
-                if (e == hipSuccess) e = launch_step_direct_far(tx, S, F, in, out, dt, grid_spacing, nullptr, st);
-            }
-            if (e == hipSuccess) e = hipEventRecord(e1, st);
+            rc = step();
+            if (rc == TF_OK && hipEventRecord(e0, st) != hipSuccess) rc = fail(TF_ERR_HIP, "autotune event");
+            for (int r = 0; r < reps && rc == TF_OK; ++r) rc = step();
+            if (rc != TF_OK) break;
+            hipError_t e = hipEventRecord(e1, st);
             if (e == hipSuccess) e = hipEventSynchronize(e1);
             float ms = 0.f;
             if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
@@ -559,17 +490,7 @@ struct tf_batch {
     int nlaunch = 0;  // non-empty stencil launches per step (classes + far rows)
 };
 
-static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
-
-size_t tf_batch_workspace_bytes(int nsims) {
-    if (nsims < 0) nsims = 0;
-    return align256((size_t)nsims * sizeof(SimDev)) + align256((size_t)(nsims + kClasses) * sizeof(int)) +
-           align256((size_t)nsims * sizeof(Ctrl)) +
-           align256((size_t)nsims * kMaxFarPerSim * kFarSplit * sizeof(FarTile)) +
-           align256((size_t)nsims * sizeof(ResetParams)) + align256((size_t)(nsims + 1) * sizeof(int)) +
-           align256((size_t)nsims * kSlotBytesPerSim);
-}
-
+size_t tf_batch_workspace_bytes(int nsims) { return plan_workspace_bytes(nsims); }
 
 int tf_set_line_mode(int enabled) {
     g_line_mode.store(enabled ? 1 : 0);
@@ -578,9 +499,9 @@ int tf_set_line_mode(int enabled) {
 
 int tf_sim_reset(const tf_sim_desc *sim, void *stream) {
     TF_REQUIRE(sim, "null sim");
-    if (int e = check_dims(sim->nx, sim->ny, sim->nz)) return e;
+    if (int e = req_dims(sim->nx, sim->ny, sim->nz)) return e;
     TF_REQUIRE(sim->buf0 && sim->buf1 && sim->ctrl, "null device pointer in sim");
-    if (int e = check_open_bc(sim->handle_boundaries, sim->nx, sim->ny, sim->nz)) return e;
+    if (int e = req_open_bc(sim->handle_boundaries, sim->nx, sim->ny, sim->nz)) return e;
     hipStream_t st = S_(stream);
     Ctrl *C = reinterpret_cast<Ctrl *>(sim->ctrl);
     TF_HIP(launch_ctrl_reset(C, sim->grid_spacing, sim->c_cfl, sim->t_final, sim->max_steps, st));
@@ -595,191 +516,84 @@ int tf_batch_create(tf_batch **out, const tf_sim_desc *sims, int nsims, void *wo
     *out = nullptr;
     TF_REQUIRE(nsims >= 1, "nsims must be >= 1");
     TF_REQUIRE(sims && workspace, "null pointer");
-    TF_REQUIRE(workspace_bytes >= tf_batch_workspace_bytes(nsims), "workspace too small: %zu < %zu",
-               workspace_bytes, tf_batch_workspace_bytes(nsims));
-    double total_cells = 0.0, max_cells = 0.0;
-    int largest = 0;
-    for (int i = 0; i < nsims; ++i) {
-        const double c = (double)sims[i].nx * sims[i].ny * sims[i].nz;
-        total_cells += c;
-        if (c > max_cells) {
-            max_cells = c;
-            largest = i;
-        }
-    }
-    // Measured loop tilings (tf_autotune_loop / tf_set_tiling) of the
-    // simulations' shapes, used when neither tile_x nor tile_y is given: the
-    // tile width of the largest simulation and each shape's own y-chunks.
+    for (int i = 0; i < nsims; ++i)
+        if (int e = req_dims(sims[i].nx, sims[i].ny, sims[i].nz)) return e;
+    // The registry's part of the plan: whole z-lines or not per shape, and,
+    // when neither tile_x nor tile_y is given, the measured loop tilings
+    // (tf_autotune_loop / tf_set_tiling): the tile width of the largest
+    // simulation and each shape's own y-chunks.
+    std::vector<unsigned char> line(nsims);
     std::vector<int> tuned_ly(nsims, 0);
     int tuned_tx = 0;
-    if (tile_x == 0 && tile_y == 0) {
-        std::lock_guard<std::mutex> lk(g_tiling_mu);
+    {
+        int largest = 0;
+        double max_cells = 0.0;
         for (int i = 0; i < nsims; ++i) {
-            auto it = g_loop_tiling.find({sims[i].nx, sims[i].ny, sims[i].nz});
-            if (it == g_loop_tiling.end()) continue;
-            tuned_ly[i] = it->second.ly;
-            if (i == largest) tuned_tx = it->second.tx;
-        }
-    }
-    // Default tile width: 8 for work that saturates HBM (fewest halo
-    // columns); 4 below that, where more resident waves per SIMD hide the
-    // latency of many short streams.
-    const int tx = tile_x != 0 ? tile_x : (tuned_tx != 0 ? tuned_tx : (total_cells >= 268435456.0 ? 8 : 4));
-    TF_REQUIRE(valid_tx(tx), "tile_x must be 0, 2, 4, 6 or 8 (got %d)", tile_x);
-    TF_REQUIRE(tile_y >= 0, "tile_y must be >= 0");
-    for (int i = 0; i < nsims; ++i) {
-        const tf_sim_desc &d = sims[i];
-        if (int e = check_dims(d.nx, d.ny, d.nz)) return e;
-        if (int e = check_fused_size(d.nx, d.ny, d.nz)) return e;
-        if (int e = check_open_bc(d.handle_boundaries, d.nx, d.ny, d.nz)) return e;
-        TF_REQUIRE(d.buf0 && d.buf1 && d.ctrl, "null device pointer in sim %d", i);
-    }
-    auto build = [&](int ly_cap, std::vector<SimDev> &tab) {
-        long total = 0;
-        for (int i = 0; i < nsims; ++i) {
-            const tf_sim_desc &d = sims[i];
-            const int ly = tile_y > 0 ? tile_y
-                                      : (tuned_ly[i] > 0 ? tuned_ly[i] : std::max(1, std::min(d.ny, ly_cap)));
-            SimDev S = make_simdev(d.nx, d.ny, d.nz, d.handle_boundaries, d.gamma, tx, ly);
-            S.buf[0] = d.buf0;
-            S.buf[1] = d.buf1;
-            S.ctrl = reinterpret_cast<Ctrl *>(d.ctrl);
-            if (use_line_tiling(d.nx, d.ny, d.nz)) {
-                int w, groups;
-                line_shape(d.nz, w, groups);
-                const int txl = line_tx(w, tx);
-                S.line_w = w;
-                S.groups = groups;
-                S.ntx = (d.nx + groups * txl - 1) / (groups * txl);
-                S.ntz = 1;
-                S.waves = S.ntx * S.nyc * w;
+            line[i] = use_line_tiling(sims[i].nx, sims[i].ny, sims[i].nz) ? 1 : 0;
+            const double c = (double)sims[i].nx * sims[i].ny * sims[i].nz;
+            if (c > max_cells) {
+                max_cells = c;
+                largest = i;
             }
-            tab[i] = S;
-            total += S.waves;
         }
-        return total;
-    };
-    // A batched step lasts at least as long as its longest wave's serial
-    // stream (rows x tile width x ~0.37 us per column-row at gfx950 clocks),
-    // and at best as long as its HBM traffic (40 B per cell at ~4.5 TB/s).
-    // Without a given tile_y, y-chunks are capped so that the longest stream
-    // stays within half the traffic time.
-    int ly_cap = 1 << 30;
-    if (tile_y == 0) {
-        const double step_us = total_cells * 40.0 / 4.5e6;
-        const double row_us = tx * 0.37;
-        ly_cap = std::max(4, (int)(0.5 * step_us / row_us) - 2);
-    }
-    std::vector<SimDev> tab(nsims);
-    long total = build(ly_cap, tab);
-    if (tile_y == 0) {
-        // Whole y-lines per wave unless the batch is too small to fill the
-        // chip several times over; then halve every simulation's y-chunks
-        // (down to 4 rows: a small batch is bound by the length of each
-        // wave's serial stream, not by the halo rows).
-        const long target = 256L * 4 * 3 * 6;
-        for (int cap = ly_cap; total < target && cap > 4;) {
-            int maxny = 1;
-            for (int i = 0; i < nsims; ++i) maxny = std::max(maxny, sims[i].ny);
-            cap = std::min(cap, maxny) / 2;
-            if (cap < 4) cap = 4;
-            total = build(cap, tab);
+        if (tile_x == 0 && tile_y == 0) {
+            std::lock_guard<std::mutex> lk(g_tiling_mu);
+            for (int i = 0; i < nsims; ++i) {
+                auto it = g_loop_tiling.find({sims[i].nx, sims[i].ny, sims[i].nz});
+                if (it == g_loop_tiling.end()) continue;
+                tuned_ly[i] = it->second.ly;
+                if (i == largest) tuned_tx = it->second.tx;
+            }
         }
     }
-    TF_REQUIRE(total <= 2000000000L, "batch too large");
+    PlanRequest rq;
+    rq.sims = sims;
+    rq.nsims = nsims;
+    rq.workspace = reinterpret_cast<uintptr_t>(workspace);
+    rq.workspace_bytes = workspace_bytes;
+    rq.tile_x = tile_x;
+    rq.tile_y = tile_y;
+    rq.tuned_ly = tuned_ly.data();
+    rq.tuned_tx = tuned_tx;
+    rq.line = line.data();
+    PlanTables P;
+    std::string err;
+    if (int rc = build_plan(rq, P, err)) return fail(rc, "%s", err.c_str());
     auto *b = new tf_batch();
     b->nsims = nsims;
-    b->tx = tx;
-    // Longest waves first (rows per wave), so that the short ones fill the
-    // tail of every launch.
-    b->order.resize(nsims);
-    for (int i = 0; i < nsims; ++i) b->order[i] = i;
-    std::stable_sort(b->order.begin(), b->order.end(), [&](int a, int c) {
-        if (tab[a].line_w != tab[c].line_w) return tab[a].line_w < tab[c].line_w;
-        const long la = (long)std::min(tab[a].ly, tab[a].ny), lc = (long)std::min(tab[c].ly, tab[c].ny);
-        return la > lc;
-    });
-    b->h_sims.resize(nsims);
-    for (int k = 0; k < nsims; ++k) b->h_sims[k] = tab[b->order[k]];
-    // classes: contiguous table ranges, each with its own prefix (waves or workgroups)
-    b->h_prefix.clear();
-    b->total_waves = 0;
-    std::vector<int> cls_prefix_at(kClasses, 0);
-    for (int c = 0, k = 0; c < kClasses; ++c) {
+    b->tx = P.tx;
+    b->total_waves = (int)P.total_waves;
+    char *const base = static_cast<char *>(workspace);
+    b->d_sims = reinterpret_cast<SimDev *>(base + P.off_sims);
+    b->d_prefix = reinterpret_cast<int *>(base + P.off_prefix);
+    b->d_status = reinterpret_cast<Ctrl *>(base + P.off_status);
+    b->d_reset = reinterpret_cast<ResetParams *>(base + P.off_reset);
+    b->d_speed_prefix = reinterpret_cast<int *>(base + P.off_speed);
+    b->d_far = reinterpret_cast<FarTile *>(base + P.off_far);
+    for (int c = 0; c < kClasses; ++c) {
+        const PlanClass &pc = P.cls[c];
         SimClass &C = b->cls[c];
-        C.w = c;
-        C.tx = c == 0 ? tx : line_tx(c, tx);
-        C.begin = k;
-        cls_prefix_at[c] = (int)b->h_prefix.size();
-        b->h_prefix.push_back(0);
-        while (k < nsims && b->h_sims[k].line_w == c) {
-            const SimDev &S = b->h_sims[k];
-            const int units = c == 0 ? S.waves : S.ntx * S.nyc;
-            b->h_prefix.push_back(b->h_prefix.back() + units);
-            b->total_waves += S.waves;
-            ++k;
-        }
-        C.count = k - C.begin;
-        C.total = b->h_prefix.back();
+        C.w = pc.w;
+        C.tx = pc.tx;
+        C.begin = pc.begin;
+        C.count = pc.count;
+        C.total = pc.total;
+        C.d_prefix = b->d_prefix + pc.prefix_at;
     }
-    char *p = static_cast<char *>(workspace);
-    b->d_sims = reinterpret_cast<SimDev *>(p);
-    p += align256((size_t)nsims * sizeof(SimDev));
-    b->d_prefix = reinterpret_cast<int *>(p);
-    for (int c = 0; c < kClasses; ++c) b->cls[c].d_prefix = b->d_prefix + cls_prefix_at[c];
-    p += align256((size_t)(nsims + kClasses) * sizeof(int));
-    b->d_status = reinterpret_cast<Ctrl *>(p);
-    p += align256((size_t)nsims * sizeof(Ctrl));
-    b->d_reset = reinterpret_cast<ResetParams *>(p);
-    p += align256((size_t)nsims * sizeof(ResetParams));
-    b->d_speed_prefix = reinterpret_cast<int *>(p);
-    p += align256((size_t)(nsims + 1) * sizeof(int));
-    char *const tables_end = p;  // sims, prefix, status, reset, speed prefix: one upload
-    b->d_far = reinterpret_cast<FarTile *>(p);
-    p += align256((size_t)nsims * kMaxFarPerSim * kFarSplit * sizeof(FarTile));
-    uint32_t *d_maxp = reinterpret_cast<uint32_t *>(p);
-    for (int k = 0; k < nsims; ++k) b->h_sims[k].maxp = d_maxp + (size_t)k * (kSlotBytesPerSim / sizeof(uint32_t));
-    std::vector<ResetParams> rp(nsims);
-    std::vector<int> sp(nsims + 1, 0);
-    int64_t speed_blocks = 0;
-    for (int k = 0; k < nsims; ++k) {
-        const tf_sim_desc &d = sims[b->order[k]];
-        rp[k] = ResetParams{d.grid_spacing, d.c_cfl, d.t_final, d.max_steps};
-        speed_blocks += ((int64_t)d.nx * d.ny * d.nz + kSpeedCellsPerBlock - 1) / kSpeedCellsPerBlock;
-        if (speed_blocks > 2000000000) {
-            delete b;
-            return fail(TF_ERR_ARG, "batch too large");
-        }
-        sp[k + 1] = (int)speed_blocks;
-    }
-    b->speed_blocks = (int)speed_blocks;
-    std::vector<FarTile> far;
-    for (int k = 0; k < nsims; ++k) {
-        const size_t before = far.size();
-        far_tiles(b->h_sims[k], tx, k, far, true);
-        if (far.size() > before) b->far_max_ntz = std::max(b->far_max_ntz, b->h_sims[k].ntz);
-    }
-    b->nfar = (int)far.size();
-    b->nlaunch = (b->nfar > 0 ? 1 : 0);
-    for (int c = 0; c < kClasses; ++c) b->nlaunch += b->cls[c].count > 0 ? 1 : 0;
+    b->h_sims = P.sims;
+    b->h_prefix = P.prefix;
+    b->order = P.order;
+    b->nfar = (int)P.far.size();
+    b->far_max_ntz = P.far_max_ntz;
+    b->speed_blocks = P.speed_blocks;
+    b->nlaunch = P.nlaunch;
+    // The tables in one upload of their host image, the far tiles (when there
+    // are any) in a second; the signal-speed slots are cleared on the device.
     hipStream_t st = S_(stream);
-    // The tables from d_sims to the end of the speed prefix in one upload,
-    // through a host image of that workspace range (the status region in it
-    // is scratch); far tiles, when there are any, in a second; the
-    // signal-speed slots are cleared on the device.
-    char *const base = reinterpret_cast<char *>(b->d_sims);
-    std::vector<char> image(static_cast<size_t>(tables_end - base), 0);
-    auto put = [&](const void *dst, const void *src, size_t bytes) {
-        if (bytes) std::memcpy(image.data() + (static_cast<const char *>(dst) - base), src, bytes);
-    };
-    put(b->d_sims, b->h_sims.data(), nsims * sizeof(SimDev));
-    put(b->d_prefix, b->h_prefix.data(), b->h_prefix.size() * sizeof(int));
-    put(b->d_reset, rp.data(), rp.size() * sizeof(ResetParams));
-    put(b->d_speed_prefix, sp.data(), sp.size() * sizeof(int));
-    hipError_t e = hipMemsetAsync(d_maxp, 0, (size_t)nsims * kSlotBytesPerSim, st);  // not in a captured graph
-    if (e == hipSuccess) e = hipMemcpyAsync(base, image.data(), image.size(), hipMemcpyHostToDevice, st);
+    hipError_t e = hipMemsetAsync(base + P.off_maxp, 0, (size_t)nsims * kSlotBytesPerSim, st);  // not in a graph
+    if (e == hipSuccess) e = hipMemcpyAsync(base, P.image.data(), P.image.size(), hipMemcpyHostToDevice, st);
     if (e == hipSuccess && b->nfar > 0)
-        e = hipMemcpyAsync(b->d_far, far.data(), far.size() * sizeof(FarTile), hipMemcpyHostToDevice, st);
+        e = hipMemcpyAsync(b->d_far, P.far.data(), P.far.size() * sizeof(FarTile), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
         delete b;
@@ -998,11 +812,11 @@ struct PlanGuard {
 };
 
 int loop_args(float *state, float *scratch, int nx, int ny, int nz, int handle_boundaries, void *workspace) {
-    if (int e = check_dims(nx, ny, nz)) return e;
-    if (int e = check_fused_size(nx, ny, nz)) return e;
+    if (int e = req_dims(nx, ny, nz)) return e;
+    if (int e = req_fused_size(nx, ny, nz)) return e;
     TF_REQUIRE(state && scratch && workspace, "null device pointer");
     TF_REQUIRE(state != scratch, "state and scratch must be different buffers");
-    if (int e = check_open_bc(handle_boundaries, nx, ny, nz)) return e;
+    if (int e = req_open_bc(handle_boundaries, nx, ny, nz)) return e;
     return TF_OK;
 }
 

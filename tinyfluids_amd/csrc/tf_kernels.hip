@@ -455,28 +455,16 @@ __device__ __forceinline__ uint32_t fused_line(const float *in_, float *out_, in
     return stream_tile<TX>(T, g, yoff0);
 }
 
-// Tile order of consecutive waves: z-tiles fastest, then x-tiles, then
-// y-chunks. The four z-tiles of a workgroup share their boundary lines in
-// L1/L2; orders with x-tiles or y-chunks inside a workgroup measured slower.
-__device__ __forceinline__ void decode_tile(int local, int ntz, int ntx, int &xt, int &zt, int &yc) {
-    zt = local % ntz;
-    const int rest = local / ntz;
-    xt = rest % ntx;
-    yc = rest / ntx;
-}
-
-// Wave index with an XCD-aware block order. Workgroups are dispatched to the
-// 8 XCDs round-robin (block b runs on XCD b % 8), each with its own L2; this
-// gives XCD k the contiguous block range [k*Q, (k+1)*Q), so neighbouring
-// tiles, which share halo rows and cache lines, meet in the same L2.
-// Returns -1 for the padding blocks of a grid rounded up to 8*Q.
+// Wave index of a halo-tiled launch with the XCD-aware block order
+// (xcd_logical_block): -1 for the padding blocks of a grid rounded up to a
+// multiple of 8. decode_tile then orders a simulation's waves z-tiles
+// fastest, so the four z-tiles of a workgroup share their boundary lines in
+// L1/L2 (orders with x-tiles or y-chunks inside a workgroup measured slower).
 __device__ __forceinline__ int xcd_wave_index(int total_waves) {
-    constexpr int kXcd = 8, kWavesPerBlock = kBlock / kWave;
+    constexpr int kWavesPerBlock = kBlock / kWave;
     const int nblocks = (total_waves + kWavesPerBlock - 1) / kWavesPerBlock;
-    const int q = (nblocks + kXcd - 1) / kXcd;
-    const int b = (int)blockIdx.x;
-    const int lb = (b % kXcd) * q + b / kXcd;
-    if (lb >= nblocks) return -1;
+    const int lb = xcd_logical_block((int)blockIdx.x, nblocks);
+    if (lb < 0) return -1;
     const int w = lb * kWavesPerBlock + wave_id_uniform();
     return w < total_waves ? w : -1;
 }
@@ -515,16 +503,6 @@ __device__ __forceinline__ void publish_max(const SimDev &S, Ctrl *C, int par, u
         atomicMax(&C->max_bits[par ^ 1], v);
 }
 
-// Does tile (xt, yc) load or store the last (x, y) row? Loaded x-rows are
-// x0-1 .. x0+TX (the first tile's left halo wraps to nx-1), loaded y-rows
-// y0-1 .. min(y0+ly, ny) (the first chunk's lower halo wraps to ny-1).
-template <int TX>
-__device__ __forceinline__ bool touches_last_row(const SimDev &S, int xt, int yc) {
-    const bool xr = xt == 0 || xt * TX + TX >= S.nx - 1;
-    const bool yr = yc == 0 || yc * S.ly + S.ly >= S.ny - 1;
-    return xr && yr;
-}
-
 // Fused step of one grid with an explicit dt (the _evolve_state entry point).
 template <int TX>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_per_simd<TX>()))) void k_step_direct(
@@ -535,7 +513,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_pe
     Gas g{S.gamma, S.gm1, S.inv_gm1, (float)(*dt / dx)};
     int xt, zt, yc;
     decode_tile(w, S.ntz, S.ntx, xt, zt, yc);
-    if (S.far && touches_last_row<TX>(S, xt, yc)) return;  // k_step_direct_far does it
+    if (S.far && touches_last_row(S, TX, xt, yc)) return;  // k_step_direct_far does it
     const int y0 = yc * S.ly;
     uint32_t v = fused_tile<TX, false>(in, out, S.nx, S.ny, S.nz, S.open_bc, g, xt, zt, y0, min(S.ly, S.ny - y0));
     if (bits_out) {
@@ -579,7 +557,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_pe
     Gas g{S.gamma, S.gm1, S.inv_gm1, C->k};
     int xt, zt, yc;
     decode_tile(w - prefix[lo], S.ntz, S.ntx, xt, zt, yc);
-    if (S.far && touches_last_row<TX>(S, xt, yc)) return;  // k_step_sims_far does it
+    if (S.far && touches_last_row(S, TX, xt, yc)) return;  // k_step_sims_far does it
     const int y0 = yc * S.ly;
     uint32_t v = fused_tile<TX, false>(S.buf[par], S.buf[par ^ 1], S.nx, S.ny, S.nz, S.open_bc, g, xt, zt, y0,
                                        min(S.ly, S.ny - y0));
@@ -619,12 +597,8 @@ constexpr int line_waves_per_simd() { return TX <= 2 ? 4 : (TX <= 4 ? 2 : 1); }
 template <int TX>
 __device__ __forceinline__ void line_block(const SimDev *__restrict__ sims, const int *__restrict__ prefix, int nsims,
                                            int W, float *lq) {
-    constexpr int kXcd = 8;
-    const int nblocks = prefix[nsims];
-    const int q = (nblocks + kXcd - 1) / kXcd;
-    const int b = (int)blockIdx.x;
-    const int lb = (b % kXcd) * q + b / kXcd;
-    if (lb >= nblocks) return;  // whole workgroup: uniform, before any barrier
+    const int lb = xcd_logical_block((int)blockIdx.x, prefix[nsims]);
+    if (lb < 0) return;  // whole workgroup: uniform, before any barrier
     int lo = 0, hi = nsims;
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
@@ -635,8 +609,8 @@ __device__ __forceinline__ void line_block(const SimDev *__restrict__ sims, cons
     if (!C->active) return;  // uniform over the workgroup
     const int par = (int)(C->n_iter & 1);
     Gas g{S.gamma, S.gm1, S.inv_gm1, C->k};
-    const int local = lb - prefix[lo];
-    const int xt = local % S.ntx, yc = local / S.ntx;
+    int xt, yc;
+    decode_line_block(lb - prefix[lo], S.ntx, xt, yc);
     uint32_t v = fused_line<TX>(S.buf[par], S.buf[par ^ 1], S.nx, S.ny, S.nz, S.open_bc, g, W, S.groups, xt, yc,
                                 S.ly, lq);
     // one device-scope atomic per workgroup: atomics to the control line

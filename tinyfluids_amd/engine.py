@@ -133,6 +133,7 @@ class JobRecord:
     admitted: float = math.nan
     finished: float = math.nan
     steps_done: int = 0
+    t: float = math.nan
     nonfinite: bool = False
     state: Optional[torch.Tensor] = None
 
@@ -285,6 +286,12 @@ class Server:
                 del x
                 g = self.group_of(s)
                 streams[g].wait_stream(upload)
+                # The buffers come from the upload stream's pool but are used
+                # on the group's stream: without this, freeing them after the
+                # group's last use (a kept state's clone still queued) would
+                # let the next upload reuse them before that use has run.
+                for t in (*sim.buf, sim.ctrl):
+                    t.record_stream(streams[g])
                 with torch.cuda.stream(streams[g]):
                     sim.reset()
                 active[j.index] = sim
@@ -305,6 +312,7 @@ class Server:
                             j = jobs[i]
                             sim = active.pop(i)
                             j.finished = now
+                            j.t = float(st.t)
                             j.nonfinite = bool(st.nonfinite)
                             if self.keep_states:
                                 j.state = sim.state(st).clone()
@@ -356,10 +364,11 @@ def serve_plain(specs: Sequence[SimSpec], time_scale: float = 1.0, device=None,
             time.sleep(j.arrival - now)
         j.admitted = time.perf_counter() - t0
         x = torch.from_numpy(j.spec.initial_state()).to(device)
-        state, _, nonfinite = run_steps(x, j.spec.dx, j.spec.gamma, j.spec.steps)
+        state, t, nonfinite = run_steps(x, j.spec.dx, j.spec.gamma, j.spec.steps)
         torch.cuda.synchronize()
         j.finished = time.perf_counter() - t0
         j.steps_done = j.spec.steps
+        j.t = t
         j.nonfinite = nonfinite
         if keep_states:
             j.state = state
