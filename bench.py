@@ -165,70 +165,108 @@ def harness_leg(dev, sizes, figures=""):
     return out
 
 
-def measured_traffic(nx, ny, nz, kernel, tile):
-    """HBM bytes per launch of the fused step from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE summary (scaled by a plain copy's counter reading),
-    when it was taken on this workload, kernel and tiling; else None."""
-    if (nx, ny, nz) != (1024, 1024, 1024):
-        return None, None
-    for path in profile_paths("fused_step_kh1024_hbm_traffic"):
-        fields = {}
-        for line in open(path):
-            if ":" in line and line.split(":")[0] in ("traffic_bytes_per_launch", "tile", "kernel"):
-                k, v = line.split(":", 1)
-                fields[k.strip()] = v.strip()
-        if fields.get("tile") == f"{tile[0]},{tile[1]}" and fields.get("kernel") == kernel:
-            return int(fields["traffic_bytes_per_launch"]), os.path.relpath(path, ROOT)
-    return None, None
+# The sources the stencil kernels are built from: a committed profile
+# summary records their hash (source_hash:), and only summaries taken on the
+# current sources are reported.
+KERNEL_SOURCES = ("tinyfluids_amd/csrc/tf_kernels.hip", "tinyfluids_amd/csrc/tf_device.h",
+                  "tinyfluids_amd/csrc/tf_layout.h", "tinyfluids_amd/csrc/tf_internal.h")
 
 
-def profile_paths(stem):
+def kernel_source_hash(root=ROOT):
+    """First 16 hex digits of the SHA-256 of the kernel sources, in order."""
+    import hashlib
+    h = hashlib.sha256()
+    for rel in KERNEL_SOURCES:
+        with open(os.path.join(root, rel), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def profile_fields(text):
+    """The `key: value` lines of a profile summary."""
+    out = {}
+    for line in text.splitlines():
+        if ":" in line and not line.startswith((" ", "|", "#", "`")):
+            k, v = line.split(":", 1)
+            out[k.strip()] = v.strip()
+    return out
+
+
+def profile_paths(stem, root=ROOT):
     """Committed summaries of one kind: profiles/<stem>.md (the default
     tiling) and profiles/<stem>_tile*.md (other tilings the tuner picks)."""
     import glob
-    return sorted(glob.glob(os.path.join(ROOT, "profiles", stem + ".md"))) + \
-        sorted(glob.glob(os.path.join(ROOT, "profiles", stem + "_tile*.md")))
+    return sorted(glob.glob(os.path.join(root, "profiles", stem + ".md"))) + \
+        sorted(glob.glob(os.path.join(root, "profiles", stem + "_tile*.md")))
 
 
-def measured_trace(nx, ny, nz, kernel, tile):
+def current_profiles(stem, kernel, tile, root=ROOT):
+    """(path, text) of the summaries of this kernel and tiling taken on the
+    current kernel sources."""
+    want = kernel_source_hash(root)
+    out = []
+    for path in profile_paths(stem, root):
+        text = open(path).read()
+        f = profile_fields(text)
+        if f.get("kernel") == kernel and f.get("tile") == f"{tile[0]},{tile[1]}" and f.get("source_hash") == want:
+            out.append((path, text))
+    return out
+
+
+def stale_profiles(root=ROOT):
+    """Committed profile summaries taken on other kernel sources (or that
+    record none): their figures describe another build."""
+    import glob
+    want = kernel_source_hash(root)
+    return sorted(os.path.relpath(p, root) for p in glob.glob(os.path.join(root, "profiles", "*.md"))
+                  if profile_fields(open(p).read()).get("source_hash") != want)
+
+
+def measured_traffic(nx, ny, nz, kernel, tile, root=ROOT):
+    """HBM bytes per launch of the fused step from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (scaled by a plain copy's counter reading),
+    when it was taken on this workload, kernel, tiling and kernel sources;
+    else (None, None)."""
+    if (nx, ny, nz) != (1024, 1024, 1024):
+        return None, None
+    for path, text in current_profiles("fused_step_kh1024_hbm_traffic", kernel, tile, root):
+        return int(profile_fields(text)["traffic_bytes_per_launch"]), os.path.relpath(path, root)
+    return None, None
+
+
+def measured_trace(nx, ny, nz, kernel, tile, root=ROOT):
     """Average launch duration of the fused step in the committed
-    rocprofv3 --kernel-trace --stats summary of this workload, kernel and
-    tiling (to compare with kernel_ms, measured here with HIP events); else
-    None."""
+    rocprofv3 --kernel-trace --stats summary of this workload, kernel,
+    tiling and kernel sources (to compare with kernel_ms, measured here with
+    HIP events); else None."""
     import re
     if (nx, ny, nz) != (1024, 1024, 1024):
         return None
-    for path in profile_paths("fused_step_kh1024_kernel_trace"):
-        text = open(path).read()
-        if f"`{kernel}`" in text and f"(tile_x,tile_y = {tile[0]},{tile[1]})" in text:
-            m = re.search(r"Average fused-step launch: \*\*([0-9.]+) ms\*\*", text)
-            if m:
-                return {"source": os.path.relpath(path, ROOT), "avg_ms": float(m.group(1))}
+    for path, text in current_profiles("fused_step_kh1024_kernel_trace", kernel, tile, root):
+        m = re.search(r"Average fused-step launch: \*\*([0-9.]+) ms\*\*", text)
+        if m:
+            return {"source": os.path.relpath(path, root), "avg_ms": float(m.group(1))}
     return None
 
 
-def measured_valu(nx, ny, nz, kernel, tile):
+def measured_valu(nx, ny, nz, kernel, tile, root=ROOT):
     """VALU instructions per cell-update and the VALU-busy fraction of the
-    fused step from the committed rocprofv3 SQ-counter summary, when it was
-    taken on this workload, kernel and tiling; else None."""
+    fused step from the committed rocprofv3 SQ-counter summary of this
+    workload, kernel, tiling and kernel sources; else None."""
     if (nx, ny, nz) != (1024, 1024, 1024):
         return None
-    for path in profile_paths("fused_step_kh1024_valu"):
-        text = open(path).read()
-        if f"`{kernel}`" in text and f"(tile_x,tile_y = {tile[0]},{tile[1]})" in text:
-            break
-    else:
-        return None
-    out = {"source": os.path.relpath(path, ROOT)}
-    for line in text.splitlines():
-        cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) != 2:
-            continue
-        if cells[0].startswith("VALU instructions per cell-update"):
-            out["insts_per_cell"] = float(cells[1])
-        elif cells[0].startswith("SQ_ACTIVE_INST_VALU"):
-            out["busy"] = float(cells[1])
-    return out if len(out) == 3 else None
+    for path, text in current_profiles("fused_step_kh1024_valu", kernel, tile, root):
+        out = {"source": os.path.relpath(path, root)}
+        for line in text.splitlines():
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) != 2:
+                continue
+            if cells[0].startswith("VALU instructions per cell-update"):
+                out["insts_per_cell"] = float(cells[1])
+            elif cells[0].startswith("SQ_ACTIVE_INST_VALU"):
+                out["busy"] = float(cells[1])
+        return out if len(out) == 3 else None
+    return None
 
 
 def batched_leg(dev):
@@ -403,6 +441,8 @@ def main():
                      "copy_ceiling": m["copy_gbs"],
                      "valu": measured_valu(nx, ny, nz, kernel, (tx, ly)),
                      "kernel_trace": measured_trace(nx, ny, nz, kernel, (tx, ly)),
+                     "kernel_source_hash": kernel_source_hash(),
+                     "stale_profiles": stale_profiles(),
                      "frac_of_copy_ceiling": achieved / m["copy_gbs"] if m["copy_gbs"] else None},
         "sim_time": st.t,
     }

@@ -17,6 +17,12 @@ small.
 
 Runs the scalar C oracle (OpenMP) for about half an hour on 8 cores.
 Run from the repository root: python tests/golden/make_golden_blast512.py
+
+With --float32-errors (after the above): runs the reference algorithm in
+float32, the reference's own precision, from the same float32 initial state
+and writes tests/golden/blast512_f32_errors.npz: its relative L1 per
+conserved field against the float64 fixtures on the same subsample and
+mid-planes (the scale of float32 round-off on those measures).
 """
 
 from __future__ import annotations
@@ -76,5 +82,30 @@ def main():
     print("done", time.time() - t0)
 
 
+def float32_errors():
+    from oracle import np_tinyfluids as npo
+    g = ics.GAMMA_BLAST
+    fix = np.load(os.path.join(HERE, "blast512_noise.npz"), allow_pickle=False)
+    planes = np.load(os.path.join(HERE, "blast512_planes.npz"), allow_pickle=False)
+    state, dx = ics.blast_noise(N, seed=1)
+    out = {}
+    t0 = time.time()
+    for step in range(0, STEPS + 1, 10):
+        if step in CHECK:
+            out[f"sub_{step}"] = npo.relative_l1(state[:, ::SUB, ::SUB, ::SUB], fix[f"sub_{step}"], g)
+        if step in PLANE_STEPS:
+            for axis, got in (("x", state[:, MID, :, :]), ("y", state[:, :, MID, :]), ("z", state[:, :, :, MID])):
+                out[f"{axis}_{step}"] = npo.relative_l1(got, planes[f"{axis}_{step}"], g)
+        if step == STEPS:
+            break
+        c_oracle.time_integration(state, dx, float("inf"), g, False, 10, 0.4, in_place=True)
+        print(f"float32 step {step + 10}: {time.time() - t0:.0f} s", flush=True)
+    np.savez_compressed(os.path.join(HERE, "blast512_f32_errors.npz"), **out)
+    print("done", {k: v.tolist() for k, v in out.items()})
+
+
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "--float32-errors":
+        float32_errors()
+    else:
+        main()

@@ -21,9 +21,10 @@ blasts and shear layers, all arriving within 45 ms, so they overlap):
     to 8.7e-4 away from float64 on the small m_y field of a 1742-step shear
     layer. The bound per conserved field is therefore max(1e-5, 2 x the
     relative L1 of the float32 reference against float64), computed here by
-    the oracle in both precisions; fields that are zero in the oracle must be
-    exactly zero, and t must match within max(1e-6, 2 x the float32
-    reference's relative t error).
+    the oracle in both precisions: the device's float32 arithmetic must be
+    about as accurate as the reference algorithm in its own precision. Fields
+    that are zero in the oracle must be exactly zero, and t must match within
+    max(1e-6, 2 x the float32 reference's relative t error).
 
 The reference's loop semantics these follow: time_integration.py:63-92.
 """
@@ -38,6 +39,7 @@ from tinyfluids_amd import ics
 pytestmark = pytest.mark.gpu
 DEV = torch.device("cuda:0")
 TOL = 1e-5
+RELAX = 2.0  # bound on long runs: this many times the float32 reference's own error
 
 
 def oracle_for(spec):
@@ -60,8 +62,9 @@ def check_against_oracle(spec, got, t, relaxed):
     if relaxed:
         ref32, t32, _ = c_oracle.run_steps(s0.astype(np.float32), spec.dx, spec.gamma, spec.steps)
         err32 = npo.relative_l1(ref32, ref, spec.gamma)
-        bound = np.maximum(TOL, 2.0 * err32)
-        tbound = max(1e-6, 2.0 * abs(t32 - t_ref) / t_ref)
+        bound = np.maximum(TOL, RELAX * err32)
+        tbound = max(1e-6, RELAX * abs(t32 - t_ref) / t_ref)
+        print(f"  device / float32-reference error: {err / np.where(err32 > 0, err32, 1)}", flush=True)
     print(f"{spec.kind} {spec.shape} {spec.steps} steps: rel L1 {err} bound {bound}, t {t} vs {t_ref}", flush=True)
     zero = np.abs(c_oracle.prim_to_cons(ref, spec.gamma)).reshape(5, -1).sum(axis=1) == 0
     assert np.all(err[zero] == 0.0), err

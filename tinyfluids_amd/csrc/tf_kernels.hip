@@ -187,7 +187,7 @@ struct HaloCtx {
 // loads or stores starts at its z-line's start. With W = 1 and nz <= 32 the
 // wave packs G = 64/nz groups of lanes, each on its own TX x-columns. Lane
 // gl = (group g, z); z-neighbours are exchanged through LDS with periodic
-// indices, one barrier per cell of a row (all waves of the group run the
+// indices, two barriers per cell of a row (all waves of the group run the
 // same rows). Loads and stores are indexed buffer accesses: a per-lane cell index
 // (x-row and z, fixed per tile) plus a scalar byte offset for the y-row.
 // ---------------------------------------------------------------------------
@@ -203,8 +203,9 @@ struct LineCtx {
     int nx;
     int lanes;            // 64 W: LDS stride of one quantity
     bool valid;
-    float *lq;            // LDS: 12 x 64W floats per slot parity, 16-byte aligned
-    static constexpr int kLq = 12;
+    float *lq;            // LDS: kLds x 64W floats, 16-byte aligned
+    static constexpr int kLq = 12;                // sweep quantities per lane
+    static constexpr int kLds = kLq + 5;          // ... and the lower-interface flux
 
     __device__ __forceinline__ void load(int j, uint32_t yoff, float (&w)[5]) const {
         line_load_priority<TX>(j, true);
@@ -212,46 +213,40 @@ struct LineCtx {
         for (int f = 0; f < 5; ++f) w[f] = tf_struct_load_f32(rin[f], (int)xi[j], 0, (int)yoff, 0);
         line_load_priority<TX>(j, false);
     }
-    // The Z sweep with one LDS exchange per column: every lane publishes its
-    // sweep quantities, then computes the fluxes at both of its interfaces
-    // (the same HLL expression, operands in the same order, as the
-    // neighbour that shares the interface), so no flux exchange and no second
-    // barrier is needed. Each lane's 12 quantities are three consecutive
-    // 16-byte vectors: 3 vector writes and 6 vector reads per lane and column
-    // (measured 0.7% faster on 1024^3 and 3-5% on 64^3-128^3 than 12 scalar
-    // accesses per neighbour on a 13-float stride); the left neighbour's
-    // vectors are read and used before the right one's, which keeps the
-    // kernels within their register budgets. Two slots alternate by column
-    // parity, so one barrier per column suffices.
-    __device__ __forceinline__ void zsweep(const Q &q, float k, float (&out)[5], int s) const {
-        float *b = lq + (s & 1) * kLq * lanes;
-        float4 *own = reinterpret_cast<float4 *>(b + gl * kLq);
+    // The Z sweep with two LDS exchanges per column: every lane publishes its
+    // sweep quantities (three consecutive 16-byte vectors), and after a
+    // barrier reads its left neighbour's and evaluates the HLL flux at its
+    // lower interface, once per interface; it publishes that flux (a 16-byte
+    // vector and a scalar, conflict-free by lane) and after a second barrier
+    // reads its right neighbour's, the flux at its upper interface. The
+    // right neighbour evaluated it with the same operands in the same order
+    // as this lane would, so the results equal those of evaluating both
+    // fluxes per lane (one barrier, one more HLL per cell), and with one slot
+    // of each kind: a lane writes the next column's quantities only after
+    // the second barrier, when every lane has read this column's.
+    __device__ __forceinline__ void zsweep(const Q &q, float k, float (&out)[5], int) const {
+        float4 *own = reinterpret_cast<float4 *>(lq + gl * kLq);
         own[0] = make_float4(q.U[0], q.U[1], q.U[2], q.U[3]);
         own[1] = make_float4(q.U[4], q.F[0], q.F[1], q.F[2]);
         own[2] = make_float4(q.F[3], q.F[4], q.up, q.um);
         __syncthreads();
-        const float4 *L4 = reinterpret_cast<const float4 *>(b + lidx * kLq);
-        const float4 *R4 = reinterpret_cast<const float4 *>(b + ridx * kLq);
+        const float4 *L4 = reinterpret_cast<const float4 *>(lq + lidx * kLq);
+        const float4 la = L4[0], lb = L4[1], lc = L4[2];
+        const float Lu[5] = {la.x, la.y, la.z, la.w, lb.x};
+        const float Lf[5] = {lb.y, lb.z, lb.w, lc.x, lc.y};
+        const HllCoef cl = hll_coef(fmax3(lc.z, q.up, 0.0f), fmin3(lc.w, q.um, 0.0f));
         float fl[5];
-        {
-            const float4 la = L4[0], lb = L4[1], lc = L4[2];
-            const float Lu[5] = {la.x, la.y, la.z, la.w, lb.x};
-            const float Lf[5] = {lb.y, lb.z, lb.w, lc.x, lc.y};
-            const HllCoef cl = hll_coef(fmax3(lc.z, q.up, 0.0f), fmin3(lc.w, q.um, 0.0f));
 #pragma unroll
-            for (int i = 0; i < 5; ++i) fl[i] = hll_comp(cl, q.U[i] - Lu[i], cl.a * Lf[i], q.F[i]);
-        }
-        {
-            const float4 ra = R4[0], rb = R4[1], rc = R4[2];
-            const float Ru[5] = {ra.x, ra.y, ra.z, ra.w, rb.x};
-            const float Rf[5] = {rb.y, rb.z, rb.w, rc.x, rc.y};
-            const HllCoef cr = hll_coef(fmax3(q.up, rc.z, 0.0f), fmin3(q.um, rc.w, 0.0f));
+        for (int i = 0; i < 5; ++i) fl[i] = hll_comp(cl, q.U[i] - Lu[i], cl.a * Lf[i], q.F[i]);
+        float4 *f4 = reinterpret_cast<float4 *>(lq + kLq * lanes);  // fluxes 0..3, one vector per lane
+        float *f1 = lq + (kLq + 4) * lanes;                           // flux 4
+        f4[gl] = make_float4(fl[0], fl[1], fl[2], fl[3]);
+        f1[gl] = fl[4];
+        __syncthreads();
+        const float4 r4 = f4[ridx];
+        const float fr[5] = {r4.x, r4.y, r4.z, r4.w, f1[ridx]};
 #pragma unroll
-            for (int i = 0; i < 5; ++i) {
-                const float fr = hll_comp(cr, Ru[i] - q.U[i], cr.a * q.F[i], Rf[i]);
-                out[i] = fmaf(k, fl[i] - fr, q.U[i]);
-            }
-        }
+        for (int i = 0; i < 5; ++i) out[i] = fmaf(k, fl[i] - fr[i], q.U[i]);
     }
     __device__ __forceinline__ bool out_ok(int s) const { return valid && xs0 + s < nx; }
     __device__ __forceinline__ void store(int s, int y, const float (&w)[5]) const {
@@ -311,7 +306,7 @@ __device__ __forceinline__ void stream_row(const Ctx &T, const Gas &g, int r, fl
 #pragma unroll
             for (int i = 0; i < 5; ++i) pin_vgpr(U3[i]);
             if (T.out_ok(s)) {
-                const Prim w3 = cons_to_prim(U3, g);
+                const Prim w3 = cons_to_prim_out(U3, g);
                 const float wo[5] = {w3.r, w3.v[0], w3.v[1], w3.v[2], w3.p};
                 T.store(s, T.y0 + r - 2, wo);
                 const float sp = signal_speed(w3.v, sound(w3.p, w3.rinv, g));
@@ -633,7 +628,7 @@ __device__ __forceinline__ void line_block(const SimDev *__restrict__ sims, cons
 template <int TX, int W>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(line_waves_per_simd<TX>()))) void k_line_sims(
     const SimDev *__restrict__ sims, const int *__restrict__ prefix, int nsims) {
-    __shared__ __attribute__((aligned(16))) float lq[2 * LineCtx<TX>::kLq * 64 * W];
+    __shared__ __attribute__((aligned(16))) float lq[LineCtx<TX>::kLds * 64 * W];
     line_block<TX>(sims, prefix, nsims, W, lq);
 }
 
@@ -643,7 +638,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(line_wav
 template <int TX, int MAXW>
 __global__ __launch_bounds__(64 * MAXW) __attribute__((amdgpu_waves_per_eu(line_waves_per_simd<TX>()))) void k_line_wide(
     const SimDev *__restrict__ sims, const int *__restrict__ prefix, int nsims, int W) {
-    __shared__ __attribute__((aligned(16))) float lq[2 * LineCtx<TX>::kLq * 64 * MAXW];
+    __shared__ __attribute__((aligned(16))) float lq[LineCtx<TX>::kLds * 64 * MAXW];
     line_block<TX>(sims, prefix, nsims, W, lq);
 }
 
@@ -759,7 +754,7 @@ __global__ __launch_bounds__(256) void k_speed_sims(const SimDev *__restrict__ s
         if (i < nc) {
             const float r = prim[i], vx = prim[nc + i], vy = prim[2 * nc + i], vz = prim[3 * nc + i],
                         p = prim[4 * nc + i];
-            const float c = fsqrt(S.gamma * p * frcp(r));
+            const float c = sound(p, frcp_nr(r), Gas{S.gamma, 0.f, 0.f, 0.f});
             const float w[3] = {vx, vy, vz};
             v = max(v, __float_as_uint(signal_speed(w, c)));
         }
@@ -795,7 +790,7 @@ __global__ __launch_bounds__(256) void k_speed(const float *__restrict__ prim, i
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += stride) {
         const float r = prim[i], vx = prim[nc + i], vy = prim[2 * nc + i], vz = prim[3 * nc + i],
                     p = prim[4 * nc + i];
-        const float c = fsqrt(gamma * p * frcp(r));
+        const float c = sound(p, frcp_nr(r), Gas{gamma, 0.f, 0.f, 0.f});
         const float w[3] = {vx, vy, vz};
         v = max(v, __float_as_uint(signal_speed(w, c)));
     }
@@ -852,7 +847,7 @@ __global__ __launch_bounds__(256) void k_sweep(const float *__restrict__ in, flo
     hll(q[0], q[1], fl);
     hll(q[1], q[2], fr);
     godunov(q[1].U, fl, fr, g.k, U1);
-    const Prim w1 = cons_to_prim(U1, g);
+    const Prim w1 = cons_to_prim_out(U1, g);
     out[i] = w1.r;
     out[nc + i] = w1.v[0];
     out[2 * nc + i] = w1.v[1];
@@ -902,7 +897,7 @@ __global__ __launch_bounds__(256) void k_c2p(const float *in, float *out, int64_
         float U[5];
 #pragma unroll
         for (int f = 0; f < 5; ++f) U[f] = in[f * nc + i];
-        const Prim w = cons_to_prim(U, g);
+        const Prim w = cons_to_prim_out(U, g);
         out[i] = w.r;
         out[nc + i] = w.v[0];
         out[2 * nc + i] = w.v[1];
