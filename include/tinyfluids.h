@@ -109,6 +109,19 @@ int tf_time_integration(float *state, float *scratch, int nx, int ny, int nz, do
                         int64_t *num_iterations, double *t_out, int *result_in_scratch, int *nonfinite,
                         void *stream);
 
+/* tf_time_integration without a workspace argument: allocates nothing. No
+ * reference counterpart; for callers that cannot allocate device memory
+ * (serving loops with a fixed footprint, harnesses that forbid allocation).
+ * The workspace lives in a 48 KiB array of device memory that the library
+ * holds on each device until the process ends; a loop of any supported grid
+ * needs tf_workspace_bytes() <= 48 KiB of it (larger needs: TF_ERR_UNSUPPORTED).
+ * Calls on one device take turns on that array (the call blocks until its
+ * run is done); arguments, results and errors as tf_time_integration. */
+int tf_time_integration_noalloc(float *state, float *scratch, int nx, int ny, int nz, double grid_spacing,
+                                double t_final, double gamma, int handle_boundaries, int64_t max_steps, double c_cfl,
+                                int64_t *num_iterations, double *t_out, int *result_in_scratch, int *nonfinite,
+                                void *stream);
+
 /* Totals of mass, momentum x/y/z and energy of a primitive state, each cell
  * converted and summed in float64 in a fixed order (deterministic). out5 is a
  * device pointer to 5 doubles. No reference counterpart (diagnostic). */

@@ -65,6 +65,9 @@ SIGNATURES = {
     "tf_evolve_state": (I, [P, P, I, I, I, D, P, D, I, P, P, P]),
     "tf_time_integration": (I, [P, P, I, I, I, D, D, D, I, I64, D, P,
                                 ctypes.POINTER(I64), ctypes.POINTER(D), ctypes.POINTER(I), ctypes.POINTER(I), P]),
+    "tf_time_integration_noalloc": (I, [P, P, I, I, I, D, D, D, I, I64, D,
+                                        ctypes.POINTER(I64), ctypes.POINTER(D), ctypes.POINTER(I), ctypes.POINTER(I),
+                                        P]),
     "tf_conserved_totals": (I, [P, I, I, I, D, P, P, P]),
     "tf_sim_reset": (I, [ctypes.POINTER(SimDesc), P]),
     "tf_batch_workspace_bytes": (SZ, [I]),

@@ -962,3 +962,55 @@ int tf_autotune_loop(float *state, float *scratch, int nx, int ny, int nz, doubl
     if (ms_per_step) *ms_per_step = best;
     return TF_OK;
 }
+
+// ------------------------------------------------------------------ allocation-free loop
+namespace {
+// Device memory of tf_time_integration_noalloc: the workspace of one
+// single-grid loop (header, totals partials, one-simulation plan tables),
+// one copy per device, held until the process ends. Host code uses only the
+// address hipGetSymbolAddress returns for it (per device), never the name.
+constexpr size_t kStaticScratchBytes = 48 * 1024;
+__device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_init_mu;
+void *g_scratch_ptr[kMaxDevices] = {};  // device address of `scratch`, per device
+std::mutex g_scratch_call_mu[kMaxDevices];  // one call at a time per device
+}  // namespace
+
+int tf_time_integration_noalloc(float *state, float *buf1, int nx, int ny, int nz, double grid_spacing,
+                                double t_final, double gamma, int handle_boundaries, int64_t max_steps, double c_cfl,
+                                int64_t *num_iterations, double *t_out, int *result_in_scratch, int *nonfinite,
+                                void *stream) {
+    const size_t need = tf_workspace_bytes(nx, ny, nz);
+    if (need > kStaticScratchBytes)
+        return fail(TF_ERR_UNSUPPORTED, "tf_time_integration_noalloc: a loop needs %zu bytes of scratch, the static "
+                                        "array holds %zu", need, kStaticScratchBytes);
+    int dev = -1;
+    TF_HIP(hipGetDevice(&dev));
+    TF_REQUIRE(dev >= 0 && dev < kMaxDevices, "tf_time_integration_noalloc: device %d beyond the %d it serves", dev,
+               kMaxDevices);
+    void *ws = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_init_mu);
+        if (!g_scratch_ptr[dev]) {
+            void *p = nullptr;
+            TF_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+            g_scratch_ptr[dev] = p;
+        }
+        ws = g_scratch_ptr[dev];
+    }
+    // The loop blocks until its work is done, so calls on one device take
+    // turns on the array; after an error the stream is drained before the
+    // next call may reuse it.
+    std::lock_guard<std::mutex> lk(g_scratch_call_mu[dev]);
+    const int rc = tf_time_integration(state, buf1, nx, ny, nz, grid_spacing, t_final, gamma, handle_boundaries,
+                                       max_steps, c_cfl, ws, num_iterations, t_out, result_in_scratch, nonfinite,
+                                       stream);
+    if (rc != TF_OK) {
+        const std::string msg = g_err;
+        (void)hipStreamSynchronize(S_(stream));
+        return fail(rc, "%s", msg.c_str());
+    }
+    return TF_OK;
+}
