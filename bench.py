@@ -45,8 +45,9 @@ def parse():
     ap.add_argument("--tile", type=str, default="", help="force tile_x,tile_y (skips autotuning)")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-jobs", type=int, default=40)
-    ap.add_argument("--serve-rates", type=str, default="8,32,128", help="Poisson arrival rates (jobs/s) to sweep")
+    ap.add_argument("--serve-jobs", type=int, default=200)
+    ap.add_argument("--serve-rates", type=str, default="16,32,64,128", help="Poisson arrival rates (jobs/s) to sweep")
+    ap.add_argument("--serve-fcfs", action="store_true", help="also serve with the arrival-order batch policy")
     ap.add_argument("--no-harness", action="store_true")
     ap.add_argument("--harness-sizes", type=str, default="32,64,96,128,196,256,384,512,768")
     ap.add_argument("--figures", type=str, default="figures",
@@ -271,8 +272,11 @@ def measured_valu(nx, ny, nz, kernel, tile, root=ROOT):
 
 def batched_leg(dev):
     """256 uneven blasts, sides {32,48,64,96,128}^3, 100 steps each: all
-    advanced together by replays of captured HIP graphs, against the plain
-    path that runs them one after another through run_steps."""
+    advanced together, against the plain path that runs them one after
+    another through run_steps (tilings of every shape measured beforehand).
+    Three launch forms of the batch: replays of a captured 16-step graph
+    (capture inside the timed region), the whole run as one captured graph
+    (capture inside), and that graph replayed once captured (warm)."""
     import torch
 
     from tinyfluids_amd import engine, ics
@@ -280,46 +284,62 @@ def batched_leg(dev):
     specs = ics.uneven_batch(256, 100, seed=3)
     init = engine.upload(specs, dev)
     tuned = engine.tune_batch(specs, init)
-    engine.run_batched(specs[:16], init[:16], keep_states=False, tile_x=tuned["tile_x"])  # warm-up
+    tx, ty = tuned["tile_x"], tuned["tile_y"]
+    engine.run_batched(specs[:16], init[:16], keep_states=False, tile_x=tx)  # warm-up
     engine.run_plain(specs[:4], init[:4], keep_states=False)
-    b = engine.run_batched(specs, init, graph=True, tile_x=tuned["tile_x"], tile_y=tuned["tile_y"])
+    b = engine.run_batched(specs, init, graph=True, tile_x=tx, tile_y=ty)
+    w = engine.run_batched(specs, init, graph="whole", tile_x=tx, tile_y=ty, keep_states=False)
+    ww = engine.run_batched(specs, init, graph="whole", tile_x=tx, tile_y=ty, warm=True)
     p = engine.run_plain(specs, init)
-    identical = all(torch.equal(x, y) for x, y in zip(b.states, p.states))
+    identical = all(torch.equal(x, y) and torch.equal(z, y) for x, y, z in zip(b.states, p.states, ww.states))
     cells = sum(s.cells * s.steps for s in specs)
     del init
+    best = min(b.seconds, ww.seconds)
     return {
         "workload": "256 seeded blasts, sides {32,48,64,96,128}^3, 100 steps each, gamma in {1.4,5/3}",
-        "sims": len(specs), "batched_s": b.seconds, "plain_s": p.seconds,
-        "sims_per_s": len(specs) / b.seconds, "plain_sims_per_s": len(specs) / p.seconds,
-        "speedup": p.seconds / b.seconds, "cell_updates_per_s": cells / b.seconds,
+        "sims": len(specs), "batched_s": best, "plain_s": p.seconds,
+        "sims_per_s": len(specs) / best, "plain_sims_per_s": len(specs) / p.seconds,
+        "speedup": p.seconds / best, "cell_updates_per_s": cells / best,
         "plain_cell_updates_per_s": cells / p.seconds, "bitwise_identical_to_plain": identical,
-        "tile_x": tuned["tile_x"], "autotune_ms_per_step": tuned["candidates"],
-        "launch": "a captured HIP graph of 16 steps replayed 6 times, then one of 4 steps (each step: stencil "
-                  "launches per tiling class + per-simulation dt update)",
+        "launch_forms": {
+            "graph_16_step_replays_capture_timed": {"seconds": b.seconds, "sims_per_s": len(specs) / b.seconds},
+            "one_graph_of_the_run_capture_timed": {"seconds": w.seconds, "sims_per_s": len(specs) / w.seconds},
+            "one_graph_of_the_run_replayed": {"seconds": ww.seconds, "sims_per_s": len(specs) / ww.seconds},
+        },
+        "tile_x": tx, "autotune_ms_per_step": tuned["candidates"],
     }
 
 
-def serve_leg(dev, njobs, rates):
+def serve_leg(dev, njobs, rates, fcfs=False):
     """Poisson job stream (grids 32..256 per side, aspect up to 4:1, 50-2000
-    steps, blast or shear layer): continuous batching against first come,
-    first served one job at a time."""
+    steps, blast or shear layer) at each arrival rate: continuous batching
+    against first come, first served one job at a time. The rates bracket
+    the plain path's saturation (about 30 jobs/s on this stream), so the
+    served throughput is measured where arrivals do not bound it."""
+    import numpy as np
+
     from tinyfluids_amd import engine, ics
 
     out = []
     for rate in rates:
         specs = ics.serving_stream(njobs, rate, seed=4)
         r = engine.Server(hbm_budget_bytes=64 << 30).serve(specs).summary()
-        f = engine.Server(hbm_budget_bytes=64 << 30, policy="fcfs").serve(specs).summary()
         p = engine.serve_plain(specs).summary()
-        out.append({"rate_jobs_per_s": rate, "jobs": njobs,
-                    "sims_per_s": r["sims_per_s"], "p50_latency_s": r["p50_latency_s"],
-                    "p99_latency_s": r["p99_latency_s"], "max_resident": r["max_resident"],
-                    "fcfs_batched": {"sims_per_s": f["sims_per_s"], "p50_latency_s": f["p50_latency_s"],
-                                     "p99_latency_s": f["p99_latency_s"]},
-                    "plain_sims_per_s": p["sims_per_s"], "plain_p50_latency_s": p["p50_latency_s"],
-                    "plain_p99_latency_s": p["p99_latency_s"],
-                    "speedup_sims_per_s": r["sims_per_s"] / p["sims_per_s"]})
-        print(f"bench: serve rate {rate}: {out[-1]}", file=sys.stderr, flush=True)
+        row = {"rate_jobs_per_s": rate, "jobs": njobs,
+               "arrival_span_s": specs[-1].arrival,
+               "sims_per_s": r["sims_per_s"], "p50_latency_s": r["p50_latency_s"],
+               "p99_latency_s": r["p99_latency_s"], "mean_latency_s": r["mean_latency_s"],
+               "cell_updates_per_s": r["cell_updates_per_s"], "max_resident": r["max_resident"],
+               "plain_sims_per_s": p["sims_per_s"], "plain_p50_latency_s": p["p50_latency_s"],
+               "plain_p99_latency_s": p["p99_latency_s"], "plain_mean_latency_s": p["mean_latency_s"],
+               "plain_cell_updates_per_s": p["cell_updates_per_s"],
+               "speedup_sims_per_s": r["sims_per_s"] / p["sims_per_s"]}
+        if fcfs:
+            f = engine.Server(hbm_budget_bytes=64 << 30, policy="fcfs").serve(specs).summary()
+            row["fcfs_batched"] = {"sims_per_s": f["sims_per_s"], "p50_latency_s": f["p50_latency_s"],
+                                   "p99_latency_s": f["p99_latency_s"]}
+        out.append(row)
+        print(f"bench: serve rate {rate}: {row}", file=sys.stderr, flush=True)
     return out
 
 
@@ -463,7 +483,8 @@ def main():
         print(f"bench: batched {rec['batched']}", file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
     if not args.no_serve:
-        rec["serving"] = serve_leg(dev, args.serve_jobs, [float(r) for r in args.serve_rates.split(",")])
+        rec["serving"] = serve_leg(dev, args.serve_jobs, [float(r) for r in args.serve_rates.split(",")],
+                                   args.serve_fcfs)
         torch.cuda.empty_cache()
     if not args.no_cpu:
         rec["cpu_baseline"] = cpu_baseline(nx, ny, args.cpu_seconds)

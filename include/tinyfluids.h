@@ -199,6 +199,11 @@ int tf_batch_step(tf_batch *b, int steps, void *stream);
  * inside it. */
 int tf_batch_step_graph(tf_batch *b, int steps, void *stream);
 
+/* As tf_batch_step for 1..1024 steps, as one captured HIP graph of the
+ * whole run launched once: captured on first use and again when the step
+ * count changes, so repeated runs of a plan replay a single launch. */
+int tf_batch_run_graph(tf_batch *b, int steps, void *stream);
+
 /* As tf_batch_step, also recording HIP events around every stencil launch
  * on `stream`; waits for the steps and returns in *stencil_ms the summed
  * duration of the stencil launches alone (without the dt updates). */

@@ -75,6 +75,20 @@ def test_graph_replays_match_eager():
             assert torch.equal(a[i].state(sa[i]), b[i].state(sb[i])), (rep, i)
 
 
+def test_whole_run_graph_matches_eager():
+    """A whole run as one captured graph (tf_batch_run_graph), cold and
+    replayed after the states are restored, equals eager stepping."""
+    from tinyfluids_amd import engine
+    specs = specs_mixed()
+    init = engine.upload(specs, DEV)
+    ref = engine.run_batched(specs, init, graph=False)
+    for warm in (False, True):
+        b = engine.run_batched(specs, init, graph="whole", warm=warm)
+        for i, (x, y, st) in enumerate(zip(b.states, ref.states, b.statuses)):
+            assert st.n_iter == specs[i].steps and not st.active, (warm, i)
+            assert torch.equal(x, y), (warm, i)
+
+
 def test_batch_reset_equals_per_simulation_reset():
     """tf_batch_reset leaves every control block as tf_sim_reset does (t, dt,
     step count, flag, signal speed), also after steps were taken."""

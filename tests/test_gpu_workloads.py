@@ -3,7 +3,8 @@
 Batched: the seeded uneven batch of 256 blasts (sides {32,48,64,96,128}^3,
 centre U[0.2,0.8]^3, radius U[0.05,0.15], pressure ratio log-uniform 2-100,
 gamma in {1.4, 5/3}, 100 steps each; ics.uneven_batch(256, 100, seed=3)),
-advanced together by captured-graph replays:
+advanced together as one captured graph of the whole run, and by replays of
+16-step graphs:
   - every simulation bit-equal to the plain single-grid loop (run_steps);
   - one simulation of each side length against the float64 oracle: relative
     L1 <= 1e-5 per conserved field and t within 1e-6.
@@ -77,11 +78,12 @@ def test_uneven_batch_256_bitwise_and_against_oracle():
     specs = ics.uneven_batch(256, 100, seed=3)
     print("uneven batch: 256 blasts, sides {32,48,64,96,128}^3, 100 steps", flush=True)
     init = engine.upload(specs, DEV)
-    b = engine.run_batched(specs, init, graph=True)
+    b = engine.run_batched(specs, init, graph="whole", warm=True)  # the run as one replayed graph
+    c = engine.run_batched(specs, init, graph=True)                # replays of 16-step graphs
     p = engine.run_plain(specs, init)
-    for i, (x, y, st) in enumerate(zip(b.states, p.states, b.statuses)):
+    for i, (x, y, z, st) in enumerate(zip(b.states, c.states, p.states, b.statuses)):
         assert st.n_iter == 100 and not st.active and not st.nonfinite, i
-        assert torch.equal(x, y), f"sim {i} {specs[i].shape}"
+        assert torch.equal(x, z) and torch.equal(y, z), f"sim {i} {specs[i].shape}"
     first = {}
     for i, s in enumerate(specs):
         first.setdefault(s.shape[0], i)

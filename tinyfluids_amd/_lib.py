@@ -77,6 +77,7 @@ SIGNATURES = {
     "tf_batch_step": (I, [P, I, P]),
     "tf_batch_reset": (I, [P, P]),
     "tf_batch_step_graph": (I, [P, I, P]),
+    "tf_batch_run_graph": (I, [P, I, P]),
     "tf_batch_status": (I, [P, ctypes.POINTER(SimStatus), P]),
     "tf_batch_step_timed": (I, [P, I, ctypes.POINTER(ctypes.c_float), P]),
     "tf_set_line_mode": (I, [I]),

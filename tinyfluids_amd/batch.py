@@ -123,6 +123,10 @@ class Batch:
     def step_graph(self, steps: int):
         _lib.check(_lib.load().tf_batch_step_graph(self._h, int(steps), _stream(self.device)))
 
+    def run_graph(self, steps: int):
+        """``steps`` (1..1024) steps as one captured graph, launched once."""
+        _lib.check(_lib.load().tf_batch_run_graph(self._h, int(steps), _stream(self.device)))
+
     def step_timed(self, steps: int) -> float:
         """Runs ``steps`` steps, waits, returns the summed stencil-kernel ms."""
         ms = ctypes.c_float(0.0)

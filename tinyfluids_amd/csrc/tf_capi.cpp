@@ -474,11 +474,14 @@ struct tf_batch {
     int speed_blocks = 0;
     int far_max_ntz = 0;
     // Captured graphs of kGraphChunk steps (slot 0) and of a shorter
-    // remainder (slot 1), replayed by tf_batch_step_graph.
+    // remainder (slot 1), replayed by tf_batch_step_graph; slot 2 holds a
+    // whole run (tf_batch_run_graph).
     static constexpr int kGraphChunk = 16;
-    hipGraph_t graph[2] = {};
-    hipGraphExec_t exec[2] = {};
-    int graph_steps[2] = {};
+    static constexpr int kGraphSlots = 3;
+    static constexpr int kWholeRunMax = 1024;
+    hipGraph_t graph[kGraphSlots] = {};
+    hipGraphExec_t exec[kGraphSlots] = {};
+    int graph_steps[kGraphSlots] = {};
     hipStream_t capture_stream = nullptr;
     // Concurrent launches of the tiling classes of one step: side streams
     // (default priority) forked from and joined back into the caller's stream
@@ -619,7 +622,7 @@ int tf_batch_sim_tiling(const tf_batch *b, int sim, int *tile_x, int *tile_y, in
 
 int tf_batch_destroy(tf_batch *b) {
     if (!b) return TF_OK;
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < tf_batch::kGraphSlots; ++i) {
         if (b->exec[i]) (void)hipGraphExecDestroy(b->exec[i]);
         if (b->graph[i]) (void)hipGraphDestroy(b->graph[i]);
     }
@@ -738,6 +741,19 @@ int tf_batch_step_graph(tf_batch *b, int steps, void *stream) {
         if (int e = ensure_graph(b, 1, rem)) return e;
         TF_HIP(hipGraphLaunch(b->exec[1], S_(stream)));
     }
+    return TF_OK;
+}
+
+// A whole run as one graph: every step of it captured once per step count
+// and launched as a single graph; simulations that reach their end return at
+// once in the remaining steps.
+int tf_batch_run_graph(tf_batch *b, int steps, void *stream) {
+    TF_REQUIRE(b, "null batch");
+    TF_REQUIRE(steps >= 1 && steps <= tf_batch::kWholeRunMax,
+               "steps must be 1..%d for one graph (tf_batch_step_graph replays chunks of any length)",
+               tf_batch::kWholeRunMax);
+    if (int e = ensure_graph(b, 2, steps)) return e;
+    TF_HIP(hipGraphLaunch(b->exec[2], S_(stream)));
     return TF_OK;
 }
 

@@ -77,21 +77,38 @@ def tune_batch(specs: Sequence[SimSpec], initial: Sequence[torch.Tensor], candid
             "candidates": {f"{k[0]},{k[1]}": v for k, v in results.items()}}
 
 
-def run_batched(specs: Sequence[SimSpec], initial: Sequence[torch.Tensor], *, graph: bool = True, tile_x: int = 0,
-                tile_y: int = 0, keep_states: bool = True) -> BatchResult:
+def run_batched(specs: Sequence[SimSpec], initial: Sequence[torch.Tensor], *, graph=True, tile_x: int = 0,
+                tile_y: int = 0, keep_states: bool = True, warm: bool = False) -> BatchResult:
     """All simulations advanced together for max(spec.steps) batched steps
-    (each stops at its own step count). Timed from reset to completion."""
+    (each stops at its own step count). Timed from reset to completion.
+
+    graph: True replays captured graphs of 16 steps; "whole" runs every step
+    as one captured graph launched once; False launches each step eagerly.
+    warm: a first run captures the graphs, the initial states are restored,
+    and the second run is the one timed (capture excluded, as a compiled
+    program's first call would be)."""
     device = initial[0].device
     sims = [Simulation(x, s.dx, s.gamma, max_steps=s.steps, device=device) for s, x in zip(specs, initial)]
     batch = Batch(sims, tile_x=tile_x, tile_y=tile_y)
     steps = max(s.steps for s in specs)
+
+    def run():
+        batch.reset()
+        if graph == "whole":
+            batch.run_graph(steps)
+        elif graph:
+            batch.step_graph(steps)
+        else:
+            batch.step(steps)
+
+    if warm:
+        run()
+        torch.cuda.synchronize()
+        for sim, x in zip(sims, initial):
+            sim.buf[0].copy_(x)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    batch.reset()
-    if graph:
-        batch.step_graph(steps)
-    else:
-        batch.step(steps)
+    run()
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     st = batch.status()
