@@ -66,6 +66,42 @@ int tf_primitive_state_from_conserved(const float *cons, float *prim, int64_t nc
 int tf_cfl_time_step(const float *prim, int nx, int ny, int nz, double grid_spacing, double gamma, double c_cfl,
                      double *dt_out, float *max_speed_out, void *workspace, void *stream);
 
+/* Replaces speed_of_sound (fluid.py:105-117): c = sqrt(gamma p / rho) for n
+ * cells (rho, p, c: n floats each). */
+int tf_speed_of_sound(const float *rho, const float *p, float *c, int64_t n, double gamma, void *stream);
+
+/* Replaces get_wave_speeds (fluid.py:119-160): the Davis speeds of ncells
+ * interfaces between primitive states primitives_left and primitives_right
+ * ([5][ncells] each), S_R = max(u_L + c_L, u_R + c_R, 0) and
+ * S_L = min(u_L - c_L, u_R - c_R, 0) with u the velocity component
+ * flux_direction_index (1..3); NaN propagates as in the reference. */
+int tf_get_wave_speeds(const float *primitives_left, const float *primitives_right, int64_t ncells, double gamma,
+                       int flux_direction_index, float *wave_speeds_left_minus, float *wave_speeds_right_plus,
+                       void *stream);
+
+/* Replaces get_max_wave_speeds (fluid.py:162-195): max(max |S_R|, max |S_L|)
+ * over the ncells interfaces, written to the device float max_out. */
+int tf_get_max_wave_speeds(const float *primitives_left, const float *primitives_right, int64_t ncells, double gamma,
+                           int flux_direction_index, float *max_out, void *workspace, void *stream);
+
+/* Replaces _euler_flux (fluid.py:206-259): the conserved flux along
+ * flux_direction_index (1..3) of ncells primitive cells ([5][ncells]). */
+int tf_euler_flux(const float *primitive_state, float *flux, int64_t ncells, double gamma, int flux_direction_index,
+                  void *stream);
+
+/* Replaces _hll_solver (fluid.py:261-305): the HLL flux
+ * (S_R F_L - S_L F_R + S_L S_R (U_R - U_L)) / (S_R - S_L) at ncells
+ * interfaces. flux must not alias the states. */
+int tf_hll_solver(const float *primitives_left, const float *primitives_right, float *flux, int64_t ncells,
+                  double gamma, int flux_direction_index, void *stream);
+
+/* Replaces _set_along_axis (fluid.py:317-331): out = in with the slice
+ * set_index along array axis `axis` of the [5][nx][ny][nz] state (0 = the
+ * variables, 1..3 = x, y, z) replaced by the slice get_index; negative
+ * indices count from the end, as in Python. in and out must not alias. */
+int tf_set_along_axis(const float *in, float *out, int nx, int ny, int nz, int axis, int set_index, int get_index,
+                      void *stream);
+
 /* Replaces _open_boundaries(state, num_ghost_cells) (fluid.py:317-413): along
  * x, then y, then z, the first g cells take the value of cell g and then the
  * last g cells that of cell n-g-1, so out[v][x][y][z] =

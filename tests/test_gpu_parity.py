@@ -112,9 +112,14 @@ def test_sweep_along_axis(axis):
 @pytest.mark.parametrize("shape", [(4, 4, 4), (7, 9, 70), (16, 12, 62), (3, 17, 125), (9, 2, 5), (1, 6, 8)])
 @pytest.mark.parametrize("hb", [False, True])
 def test_fused_step(shape, hb):
-    if hb and min(shape) < 2:
-        pytest.skip("open boundaries need every dimension >= 2")
     s = random_state(shape, 4)
+    if hb and min(shape) < 2:
+        # open boundaries read cell 1 of every axis (the get index
+        # num_ghost_cells of _open_left_boundary, fluid.py:356-379), which an
+        # axis of one cell does not have: rejected rather than made up
+        with pytest.raises(ValueError, match="open boundaries"):
+            tf()._evolve_state(to_dev(s), 0.05, 0.004, 5 / 3, hb)
+        return
     print(f"fused step shape={shape} handle_boundaries={hb}", flush=True)
     out = tf()._evolve_state(to_dev(s), 0.05, 0.004, 5 / 3, hb)
     ref = npo._evolve_state(s.astype(np.float64), 0.05, 0.004, 5 / 3, hb)

@@ -12,6 +12,8 @@ from .fluid import (  # noqa: F401
     VAR_AXIS, X_AXIS, Y_AXIS, Z_AXIS,
     conserved_state_from_primitive, primitive_state_from_conserved, _cfl_time_step, _open_boundaries,
     _evolve_state_along_axis, _evolve_state, _evolve_state_split, conserved_totals, max_signal_speed,
+    speed_of_sound, get_wave_speeds, get_max_wave_speeds, _euler_flux, _hll_solver, _set_along_axis,
+    _open_left_boundary, _open_right_boundary,
 )
 from .time_integration import time_integration, run_steps  # noqa: F401
 

@@ -61,6 +61,12 @@ SIGNATURES = {
     "tf_primitive_state_from_conserved": (I, [P, P, I64, D, P]),
     "tf_cfl_time_step": (I, [P, I, I, I, D, D, D, P, P, P, P]),
     "tf_open_boundaries": (I, [P, P, I, I, I, I, P]),
+    "tf_speed_of_sound": (I, [P, P, P, I64, D, P]),
+    "tf_get_wave_speeds": (I, [P, P, I64, D, I, P, P, P]),
+    "tf_get_max_wave_speeds": (I, [P, P, I64, D, I, P, P, P]),
+    "tf_euler_flux": (I, [P, P, I64, D, I, P]),
+    "tf_hll_solver": (I, [P, P, P, I64, D, I, P]),
+    "tf_set_along_axis": (I, [P, P, I, I, I, I, I, I, P]),
     "tf_evolve_state_along_axis": (I, [P, P, I, I, I, D, P, D, I, P]),
     "tf_evolve_state": (I, [P, P, I, I, I, D, P, D, I, P, P, P]),
     "tf_time_integration": (I, [P, P, I, I, I, D, D, D, I, I64, D, P,

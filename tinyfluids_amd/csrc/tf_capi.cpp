@@ -51,6 +51,14 @@ hipError_t launch_c2p(const float *in, float *out, int64_t nc, const Gas &g, hip
 hipError_t launch_totals(const float *prim, int64_t nc, double gamma, double *partials, double *out5,
                          hipStream_t st);
 hipError_t launch_gather_status(const SimDev *sims, int nsims, Ctrl *dst, hipStream_t st);
+hipError_t launch_sound_speed(const float *rho, const float *p, float *c, int64_t n, const Gas &g, hipStream_t st);
+hipError_t launch_wave_speeds(const float *L, const float *R, float *sl, float *sr, int64_t n, const Gas &g, int dir,
+                              uint32_t *bits, hipStream_t st);
+hipError_t launch_euler_flux(const float *w, float *F, int64_t n, const Gas &g, int dir, hipStream_t st);
+hipError_t launch_hll_flux(const float *L, const float *R, float *F, int64_t n, const Gas &g, int dir,
+                           hipStream_t st);
+hipError_t launch_set_along_axis(const float *in, float *out, int nx, int ny, int nz, int axis, int set_index,
+                                 int get_index, hipStream_t st);
 }  // namespace tf
 
 using namespace tf;
@@ -339,6 +347,83 @@ int tf_conserved_totals(const float *prim, int nx, int ny, int nz, double gamma,
     TF_REQUIRE(prim && out5 && workspace, "null pointer");
     double *partials = reinterpret_cast<double *>(static_cast<char *>(workspace) + kWsHeaderBytes);
     TF_HIP(launch_totals(prim, (int64_t)nx * ny * nz, gamma, partials, out5, S_(stream)));
+    return TF_OK;
+}
+
+// ------------------------------------------------------------------ per-interface helpers
+int tf_speed_of_sound(const float *rho, const float *p, float *c, int64_t n, double gamma, void *stream) {
+    TF_REQUIRE(rho && p && c, "null pointer");
+    TF_REQUIRE(n >= 0, "n must be >= 0");
+    if (n == 0) return TF_OK;
+    TF_HIP(launch_sound_speed(rho, p, c, n, make_gas(gamma), S_(stream)));
+    return TF_OK;
+}
+
+int tf_get_wave_speeds(const float *primitives_left, const float *primitives_right, int64_t ncells, double gamma,
+                       int flux_direction_index, float *wave_speeds_left_minus, float *wave_speeds_right_plus,
+                       void *stream) {
+    TF_REQUIRE(primitives_left && primitives_right && wave_speeds_left_minus && wave_speeds_right_plus,
+               "null pointer");
+    TF_REQUIRE(ncells >= 0, "ncells must be >= 0");
+    TF_REQUIRE(flux_direction_index >= 1 && flux_direction_index <= 3,
+               "flux_direction_index must be 1, 2 or 3 (got %d)", flux_direction_index);
+    if (ncells == 0) return TF_OK;
+    TF_HIP(launch_wave_speeds(primitives_left, primitives_right, wave_speeds_left_minus, wave_speeds_right_plus,
+                              ncells, make_gas(gamma), flux_direction_index, nullptr, S_(stream)));
+    return TF_OK;
+}
+
+int tf_get_max_wave_speeds(const float *primitives_left, const float *primitives_right, int64_t ncells, double gamma,
+                           int flux_direction_index, float *max_out, void *workspace, void *stream) {
+    TF_REQUIRE(primitives_left && primitives_right && max_out && workspace, "null pointer");
+    TF_REQUIRE(ncells >= 1, "ncells must be >= 1 (the maximum of nothing is undefined)");
+    TF_REQUIRE(flux_direction_index >= 1 && flux_direction_index <= 3,
+               "flux_direction_index must be 1, 2 or 3 (got %d)", flux_direction_index);
+    auto *H = reinterpret_cast<WsHeader *>(workspace);
+    hipStream_t st = S_(stream);
+    TF_HIP(launch_fill_u32(&H->bits[1], 0u, 1, st));
+    TF_HIP(launch_wave_speeds(primitives_left, primitives_right, nullptr, nullptr, ncells, make_gas(gamma),
+                              flux_direction_index, &H->bits[1], st));
+    TF_HIP(launch_dt_from_bits(&H->bits[1], 1.0, 0.0, nullptr, max_out, st));
+    return TF_OK;
+}
+
+int tf_euler_flux(const float *primitive_state, float *flux, int64_t ncells, double gamma, int flux_direction_index,
+                  void *stream) {
+    TF_REQUIRE(primitive_state && flux, "null pointer");
+    TF_REQUIRE(primitive_state != flux, "primitive_state and flux must not alias");
+    TF_REQUIRE(ncells >= 0, "ncells must be >= 0");
+    TF_REQUIRE(flux_direction_index >= 1 && flux_direction_index <= 3,
+               "flux_direction_index must be 1, 2 or 3 (got %d)", flux_direction_index);
+    if (ncells == 0) return TF_OK;
+    TF_HIP(launch_euler_flux(primitive_state, flux, ncells, make_gas(gamma), flux_direction_index, S_(stream)));
+    return TF_OK;
+}
+
+int tf_hll_solver(const float *primitives_left, const float *primitives_right, float *flux, int64_t ncells,
+                  double gamma, int flux_direction_index, void *stream) {
+    TF_REQUIRE(primitives_left && primitives_right && flux, "null pointer");
+    TF_REQUIRE(flux != primitives_left && flux != primitives_right, "flux must not alias the states");
+    TF_REQUIRE(ncells >= 0, "ncells must be >= 0");
+    TF_REQUIRE(flux_direction_index >= 1 && flux_direction_index <= 3,
+               "flux_direction_index must be 1, 2 or 3 (got %d)", flux_direction_index);
+    if (ncells == 0) return TF_OK;
+    TF_HIP(launch_hll_flux(primitives_left, primitives_right, flux, ncells, make_gas(gamma), flux_direction_index,
+                           S_(stream)));
+    return TF_OK;
+}
+
+int tf_set_along_axis(const float *in, float *out, int nx, int ny, int nz, int axis, int set_index, int get_index,
+                      void *stream) {
+    if (int e = req_dims(nx, ny, nz)) return e;
+    TF_REQUIRE(in && out, "null pointer");
+    TF_REQUIRE(in != out, "in and out must not alias");
+    TF_REQUIRE(axis >= 0 && axis <= 3, "axis must be 0 (variables), 1, 2 or 3 (got %d)", axis);
+    const int n = axis == 0 ? 5 : (axis == 1 ? nx : (axis == 2 ? ny : nz));
+    const int s = set_index < 0 ? set_index + n : set_index, g = get_index < 0 ? get_index + n : get_index;
+    TF_REQUIRE(s >= 0 && s < n, "set_index %d out of range for axis %d of size %d", set_index, axis, n);
+    TF_REQUIRE(g >= 0 && g < n, "get_index %d out of range for axis %d of size %d", get_index, axis, n);
+    TF_HIP(launch_set_along_axis(in, out, nx, ny, nz, axis, s, g, S_(stream)));
     return TF_OK;
 }
 

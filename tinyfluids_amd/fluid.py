@@ -8,9 +8,11 @@ kernel through the C ABI on the current torch stream and returns a new tensor
 Like the reference's jitted functions, calls are asynchronous; synchronise
 as torch does.
 
-The reference's per-interface helpers that never leave its sweep
-(get_wave_speeds, _euler_flux, _hll_solver, fluid.py:119-305) are fused into
-the step kernels and have no separate entry point here.
+The reference's per-interface helpers (speed_of_sound, get_wave_speeds,
+get_max_wave_speeds, _euler_flux, _hll_solver, fluid.py:105-305) are fused
+into the step kernels; they are also here as stand-alone elementwise kernels,
+as are its boundary helpers (_set_along_axis, _open_left_boundary,
+_open_right_boundary, fluid.py:317-379).
 """
 
 from __future__ import annotations
@@ -118,6 +120,103 @@ def max_signal_speed(primitive_state, gamma):
     _lib.check(_lib.load().tf_cfl_time_step(
         _ptr(s), nx, ny, nz, 1.0, _scalar(gamma, "gamma"), 0.4, None, _ptr(m), _ptr(ws), _stream(s.device)))
     return m
+
+
+def _field(x, name):
+    if not isinstance(x, torch.Tensor) or x.dtype != torch.float32 or x.device.type != "cuda":
+        raise TypeError(f"{name} must be a float32 tensor on a ROCm device")
+    return x.contiguous()
+
+
+def _direction(flux_direction_index):
+    if flux_direction_index not in (VELOCITY_X_INDEX, VELOCITY_Y_INDEX, VELOCITY_Z_INDEX):
+        raise ValueError(f"flux_direction_index must be 1, 2 or 3, got {flux_direction_index}")
+    return int(flux_direction_index)
+
+
+def _pair(primitives_left, primitives_right):
+    left = check_state(primitives_left, "primitives_left")
+    right = check_state(primitives_right, "primitives_right")
+    if left.shape != right.shape:
+        raise TypeError(f"left and right states differ in shape: {tuple(left.shape)} vs {tuple(right.shape)}")
+    return left, right
+
+
+def speed_of_sound(rho, p, gamma):
+    """fluid.py:105-117: sqrt(gamma p / rho), elementwise."""
+    rho, p = _field(rho, "rho"), _field(p, "p")
+    if rho.shape != p.shape:
+        raise TypeError(f"rho and p differ in shape: {tuple(rho.shape)} vs {tuple(p.shape)}")
+    c = torch.empty_like(rho)
+    _lib.check(_lib.load().tf_speed_of_sound(_ptr(rho), _ptr(p), _ptr(c), rho.numel(), _scalar(gamma, "gamma"),
+                                             _stream(rho.device)))
+    return c
+
+
+def get_wave_speeds(primitives_left, primitives_right, gamma, flux_direction_index):
+    """fluid.py:119-160: (wave_speeds_left_minus, wave_speeds_right_plus)."""
+    left, right = _pair(primitives_left, primitives_right)
+    sl, sr = torch.empty_like(left[0]), torch.empty_like(left[0])
+    _lib.check(_lib.load().tf_get_wave_speeds(_ptr(left), _ptr(right), left[0].numel(), _scalar(gamma, "gamma"),
+                                              _direction(flux_direction_index), _ptr(sl), _ptr(sr),
+                                              _stream(left.device)))
+    return sl, sr
+
+
+def get_max_wave_speeds(primitives_left, primitives_right, gamma, flux_direction_index):
+    """fluid.py:162-195: max(max |S_R|, max |S_L|) as a 0-d float32 device tensor."""
+    left, right = _pair(primitives_left, primitives_right)
+    m = torch.empty((), dtype=torch.float32, device=left.device)
+    ws = workspace(*left.shape[1:], left.device)
+    _lib.check(_lib.load().tf_get_max_wave_speeds(_ptr(left), _ptr(right), left[0].numel(), _scalar(gamma, "gamma"),
+                                                  _direction(flux_direction_index), _ptr(m), _ptr(ws),
+                                                  _stream(left.device)))
+    return m
+
+
+def _euler_flux(primitive_state, gamma, flux_direction_index):
+    """fluid.py:206-259."""
+    s = check_state(primitive_state)
+    out = torch.empty_like(s)
+    _lib.check(_lib.load().tf_euler_flux(_ptr(s), _ptr(out), s[0].numel(), _scalar(gamma, "gamma"),
+                                         _direction(flux_direction_index), _stream(s.device)))
+    return out
+
+
+def _hll_solver(primitives_left, primitives_right, gamma, flux_direction_index):
+    """fluid.py:261-305."""
+    left, right = _pair(primitives_left, primitives_right)
+    out = torch.empty_like(left)
+    _lib.check(_lib.load().tf_hll_solver(_ptr(left), _ptr(right), _ptr(out), left[0].numel(), _scalar(gamma, "gamma"),
+                                         _direction(flux_direction_index), _stream(left.device)))
+    return out
+
+
+def _set_along_axis(primitive_state, axis, set_index, get_index):
+    """fluid.py:317-331 (axis 0..3 of the state; negative indices count from the end)."""
+    s = check_state(primitive_state)
+    _, nx, ny, nz = s.shape
+    out = torch.empty_like(s)
+    _lib.check(_lib.load().tf_set_along_axis(_ptr(s), _ptr(out), nx, ny, nz, int(axis), int(set_index),
+                                             int(get_index), _stream(s.device)))
+    return out
+
+
+def _open_right_boundary(primitive_state, num_ghost_cells, axis):
+    """fluid.py:333-354: the last num_ghost_cells slices along axis take the
+    value of the slice before them."""
+    get_index = -num_ghost_cells - 1
+    for set_index in range(-num_ghost_cells, 0):
+        primitive_state = _set_along_axis(primitive_state, axis, set_index, get_index)
+    return primitive_state
+
+
+def _open_left_boundary(primitive_state, num_ghost_cells, axis):
+    """fluid.py:356-379: the first num_ghost_cells slices along axis take the
+    value of the slice after them."""
+    for set_index in range(num_ghost_cells):
+        primitive_state = _set_along_axis(primitive_state, axis, set_index, num_ghost_cells)
+    return primitive_state
 
 
 def _open_boundaries(primitive_state, num_ghost_cells=1):
