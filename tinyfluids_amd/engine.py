@@ -246,16 +246,20 @@ class Server:
         left = max(jobs[i].spec.steps - jobs[i].steps_done for i in ids)
         return int(max(1, min(left, 1024, self.tick_ms * 1e3 / step_us)))
 
-    def serve(self, specs: Sequence[SimSpec], time_scale: float = 1.0) -> ServeReport:
+    def serve(self, specs: Sequence[SimSpec], time_scale: float = 1.0,
+              initial: Optional[Sequence[np.ndarray]] = None) -> ServeReport:
         """Serves jobs arriving at spec.arrival * time_scale seconds after the
         start (wall clock). Returns per-job records. The tilings of the
         stream's grid shapes are measured beforehand, as serve_plain does; a
         batch of one job then runs exactly like the plain path.
 
-        A job's initial state is built by a pool of host threads from the
-        moment it arrives, and uploaded on a stream of its own when the job
-        is admitted, so neither waits for the device's ticks."""
+        ``initial``: each job's initial state (host float32 arrays), as a
+        request would carry it. Without it, a job's initial state is built
+        from its spec by a pool of host threads from the moment it arrives.
+        Either way it is uploaded on a stream of its own when the job is
+        admitted, so neither waits for the device's ticks."""
         jobs = [JobRecord(i, s, s.arrival * time_scale) for i, s in enumerate(specs)]
+        self._initial = initial
         for j in jobs:
             if state_bytes(j.spec) > self.budget:
                 raise ValueError(f"job {j.index} ({j.spec.shape}) does not fit the HBM budget")
@@ -291,7 +295,10 @@ class Server:
             while pending and pending[0].arrival <= now:
                 j = pending.pop(0)
                 queue.append(j)
-                prepared[j.index] = pool.submit(j.spec.initial_state)
+                if self._initial is not None:
+                    prepared[j.index] = pool.submit(lambda a=self._initial[j.index]: a)
+                else:
+                    prepared[j.index] = pool.submit(j.spec.initial_state)
             while queue and len(active) < self.max_resident and used + state_bytes(queue[0].spec) <= self.budget:
                 if not prepared[queue[0].index].done():
                     break  # admission stays in arrival order
@@ -366,9 +373,10 @@ class Server:
 
 
 def serve_plain(specs: Sequence[SimSpec], time_scale: float = 1.0, device=None,
-                keep_states: bool = False) -> ServeReport:
+                keep_states: bool = False, initial: Optional[Sequence[np.ndarray]] = None) -> ServeReport:
     """First come, first served, one job at a time through run_steps (loop
-    tilings of the stream's grid shapes measured beforehand)."""
+    tilings of the stream's grid shapes measured beforehand). ``initial`` as
+    for Server.serve; without it each job's state is built when it starts."""
     device = device or torch.device("cuda:0")
     tune_plain(specs, device)
     jobs = [JobRecord(i, s, s.arrival * time_scale) for i, s in enumerate(specs)]
@@ -380,7 +388,8 @@ def serve_plain(specs: Sequence[SimSpec], time_scale: float = 1.0, device=None,
         if j.arrival > now:
             time.sleep(j.arrival - now)
         j.admitted = time.perf_counter() - t0
-        x = torch.from_numpy(j.spec.initial_state()).to(device)
+        x0 = initial[j.index] if initial is not None else j.spec.initial_state()
+        x = torch.from_numpy(x0).to(device)
         state, t, nonfinite = run_steps(x, j.spec.dx, j.spec.gamma, j.spec.steps)
         torch.cuda.synchronize()
         j.finished = time.perf_counter() - t0
