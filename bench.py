@@ -336,6 +336,8 @@ def serve_leg(dev, njobs, rates, fcfs=False):
                "sims_per_s": r["sims_per_s"], "p50_latency_s": r["p50_latency_s"],
                "p99_latency_s": r["p99_latency_s"], "mean_latency_s": r["mean_latency_s"],
                "cell_updates_per_s": r["cell_updates_per_s"], "max_resident": r["max_resident"],
+               "ticks": r["ticks"], "makespan_s": r["makespan_s"], "status_wait_s": r.get("status_wait_s"),
+               "plan_rebuild_s": r.get("plan_rebuild_s"),
                "plain_sims_per_s": p["sims_per_s"], "plain_p50_latency_s": p["p50_latency_s"],
                "plain_p99_latency_s": p["p99_latency_s"], "plain_mean_latency_s": p["mean_latency_s"],
                "plain_cell_updates_per_s": p["cell_updates_per_s"],

@@ -25,7 +25,7 @@ from __future__ import annotations
 
 import math
 import time
-from concurrent.futures import ThreadPoolExecutor, wait
+from concurrent.futures import FIRST_COMPLETED, ThreadPoolExecutor, wait
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Sequence
 
@@ -181,6 +181,7 @@ class ServeReport:
             "ticks": self.ticks,
             "max_resident": self.max_resident,
             "peak_state_bytes": self.peak_bytes,
+            **self.extra,
         }
 
 
@@ -199,13 +200,17 @@ class Server:
     job progresses), then the others shortest remaining work first (cells x
     steps left; ``policy="fcfs"``: arrival order) until it holds
     ``fill_cells`` cells, about the size at which one launch saturates HBM.
-    ``steps_per_tick`` > 0 fixes the steps per tick instead."""
+    ``steps_per_tick`` > 0 fixes the steps per tick instead.
+
+    Admission reserves a job's memory and slot in arrival order; its upload
+    then runs on a pool of ``upload_threads`` host threads, and the job joins
+    its group when the upload is done."""
 
     GROUP_BOUNDS = (1 << 20, 1 << 23)
 
     def __init__(self, hbm_budget_bytes: int, max_resident: int = 1024, steps_per_tick: int = 0, device=None,
                  keep_states: bool = False, tile_x: int = 0, fill_cells: int = 1 << 24, tick_ms: float = 4.0,
-                 policy: str = "shortest", prep_threads: int = 4):
+                 policy: str = "shortest", prep_threads: int = 4, upload_threads: int = 2):
         if policy not in ("shortest", "fcfs"):
             raise ValueError(f"policy must be 'shortest' or 'fcfs', got {policy!r}")
         self.policy = policy
@@ -218,6 +223,7 @@ class Server:
         self.fill_cells = int(fill_cells)
         self.tick_ms = float(tick_ms)
         self.prep_threads = max(1, int(prep_threads))
+        self.upload_threads = max(1, int(upload_threads))
 
     @classmethod
     def group_of(cls, spec: SimSpec) -> int:
@@ -256,8 +262,10 @@ class Server:
         ``initial``: each job's initial state (host float32 arrays), as a
         request would carry it. Without it, a job's initial state is built
         from its spec by a pool of host threads from the moment it arrives.
-        Either way it is uploaded on a stream of its own when the job is
-        admitted, so neither waits for the device's ticks."""
+        Either way, an admitted job's state is uploaded by a second pool of
+        host threads on a stream of its own: the serving loop never blocks on
+        a host-to-device copy, so the ticks it has queued keep the device
+        busy, and a group stream waits only for its own job's upload."""
         jobs = [JobRecord(i, s, s.arrival * time_scale) for i, s in enumerate(specs)]
         self._initial = initial
         for j in jobs:
@@ -267,18 +275,35 @@ class Server:
         ngroups = len(self.GROUP_BOUNDS) + 1
         streams = [torch.cuda.Stream(self.device) for _ in range(ngroups)]  # default priority
         upload = torch.cuda.Stream(self.device)
-        with ThreadPoolExecutor(max_workers=self.prep_threads) as pool:
-            return self._serve(jobs, streams, upload, pool)
+        # two pools: an upload waits for its job's host state, so uploads
+        # must not occupy the threads that build those states
+        with ThreadPoolExecutor(max_workers=self.prep_threads) as pool, \
+                ThreadPoolExecutor(max_workers=self.upload_threads) as uploader:
+            return self._serve(jobs, streams, upload, pool, uploader)
 
-    def _serve(self, jobs: List[JobRecord], streams, upload, pool) -> ServeReport:
+    def _upload(self, spec: SimSpec, host_state, upload) -> tuple:
+        """Runs on an upload thread: the job's buffers and control block,
+        allocated and filled on the upload stream (the copy from pageable host
+        memory blocks this thread only). Returns the simulation and an event
+        after its last upload-stream work."""
+        x = host_state.result()
+        with torch.cuda.device(self.device), torch.cuda.stream(upload):
+            sim = Simulation(x, spec.dx, spec.gamma, max_steps=spec.steps, device=self.device)
+            ev = torch.cuda.Event()
+            ev.record(upload)
+        return sim, ev
+
+    def _serve(self, jobs: List[JobRecord], streams, upload, pool, uploader) -> ServeReport:
         ngroups = len(streams)
         prepared = {}  # job index -> future of its initial state (host array)
+        uploading = {}  # job index -> future of (Simulation, event) on the upload pool
         pending = list(jobs)  # in arrival order
         queue: List[JobRecord] = []
         active: Dict[int, Simulation] = {}
         batches: List[Optional[Batch]] = [None] * ngroups
         ids: List[List[int]] = [[] for _ in range(ngroups)]
         used = peak = ticks = max_res = done = 0
+        t_wait = t_plan = 0.0  # host seconds waiting for tick outcomes / rebuilding plans
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         while done < len(jobs):
@@ -299,17 +324,19 @@ class Server:
                     prepared[j.index] = pool.submit(lambda a=self._initial[j.index]: a)
                 else:
                     prepared[j.index] = pool.submit(j.spec.initial_state)
-            while queue and len(active) < self.max_resident and used + state_bytes(queue[0].spec) <= self.budget:
-                if not prepared[queue[0].index].done():
-                    break  # admission stays in arrival order
+            # admission in arrival order: memory and a slot are reserved now,
+            # the upload runs on the upload pool
+            while (queue and len(active) + len(uploading) < self.max_resident
+                   and used + state_bytes(queue[0].spec) <= self.budget):
                 j = queue.pop(0)
-                s = j.spec
-                x = prepared.pop(j.index).result()
-                with torch.cuda.stream(upload):
-                    sim = Simulation(x, s.dx, s.gamma, max_steps=s.steps, device=self.device)
-                del x
-                g = self.group_of(s)
-                streams[g].wait_stream(upload)
+                uploading[j.index] = uploader.submit(self._upload, j.spec, prepared.pop(j.index), upload)
+                used += state_bytes(j.spec)
+                j.admitted = time.perf_counter() - t0
+            # uploaded jobs join their group
+            for i in [i for i, f in uploading.items() if f.done()]:
+                sim, ev = uploading.pop(i).result()
+                g = self.group_of(jobs[i].spec)
+                streams[g].wait_event(ev)
                 # The buffers come from the upload stream's pool but are used
                 # on the group's stream: without this, freeing them after the
                 # group's last use (a kept state's clone still queued) would
@@ -318,9 +345,7 @@ class Server:
                     t.record_stream(streams[g])
                 with torch.cuda.stream(streams[g]):
                     sim.reset()
-                active[j.index] = sim
-                used += state_bytes(s)
-                j.admitted = time.perf_counter() - t0
+                active[i] = sim
             peak = max(peak, used)
             max_res = max(max_res, len(active))
             # 3. each group's tick outcome: finished jobs leave and free their memory
@@ -328,7 +353,9 @@ class Server:
                 if not ticked[g]:
                     continue
                 with torch.cuda.stream(streams[g]):
+                    tw = time.perf_counter()
                     statuses = b.status()  # waits for this group's tick
+                    t_wait += time.perf_counter() - tw
                     now = time.perf_counter() - t0
                     for i, st in zip(ids[g], statuses):
                         jobs[i].steps_done = int(st.n_iter)
@@ -343,6 +370,7 @@ class Server:
                             used -= state_bytes(j.spec)
                             done += 1
             # 4. a group's plan is rebuilt when its chosen jobs change
+            tp = time.perf_counter()
             members = [[] for _ in range(ngroups)]
             for i in active:
                 members[self.group_of(jobs[i].spec)].append(i)
@@ -356,20 +384,22 @@ class Server:
                         ids[g] = chosen
                         if chosen:
                             batches[g] = Batch([active[i] for i in chosen], tile_x=self.tile_x)
+            t_plan += time.perf_counter() - tp
             if all(b is None for b in batches):
-                if pending and not queue:
+                if uploading:
+                    # idle: wait for an upload
+                    wait(list(uploading.values()), timeout=0.01, return_when=FIRST_COMPLETED)
+                elif pending and not queue:
                     # idle: wait for the next arrival
                     time.sleep(max(0.0, min(0.01, pending[0].arrival - (time.perf_counter() - t0))))
-                elif queue and queue[0].index in prepared:
-                    # idle: wait for the oldest queued job's initial state
-                    wait([prepared[queue[0].index]], timeout=0.01)
         for g, b in enumerate(batches):
             if b is not None:
                 with torch.cuda.stream(streams[g]):
                     b.close()
         torch.cuda.synchronize()
         makespan = time.perf_counter() - t0
-        return ServeReport(jobs, makespan, ticks, max_res, peak)
+        return ServeReport(jobs, makespan, ticks, max_res, peak,
+                           extra={"status_wait_s": t_wait, "plan_rebuild_s": t_plan})
 
 
 def serve_plain(specs: Sequence[SimSpec], time_scale: float = 1.0, device=None,
