@@ -217,6 +217,16 @@ size_t tf_batch_workspace_bytes(int nsims);
  * above 8 waves. */
 int tf_batch_create(tf_batch **out, const tf_sim_desc *sims, int nsims, void *workspace, size_t workspace_bytes,
                     int tile_x, int tile_y, void *stream);
+
+/* tf_batch_create without waiting, for callers that rebuild plans while
+ * earlier steps are still queued on the stream: the tables are uploaded on
+ * `stream` from page-locked staging memory the library keeps (and reuses
+ * once the upload has completed), and the call returns at once. The plan
+ * may be stepped on `stream` at once, or on another stream after an event
+ * recorded on `stream`. */
+int tf_batch_create_async(tf_batch **out, const tf_sim_desc *sims, int nsims, void *workspace,
+                          size_t workspace_bytes, int tile_x, int tile_y, void *stream);
+
 int tf_batch_destroy(tf_batch *b);
 
 /* The tiling a plan runs simulation `sim` (caller's order) with: tile width,
@@ -259,6 +269,18 @@ int tf_batch_reset(tf_batch *b, void *stream);
 
 /* Copies every simulation's status to host memory (synchronous). */
 int tf_batch_status(tf_batch *b, tf_sim_status *host_out, void *stream);
+
+/* tf_batch_status in two halves, for callers that keep further steps queued
+ * while they read a status. tf_batch_status_enqueue enqueues on `stream`
+ * (the stream the batch steps on) a copy of every simulation's status into
+ * `host_raw`, tf_batch_status_bytes(b) bytes of page-locked host memory (with
+ * pageable memory the copy blocks), and returns. Once the stream has passed
+ * the copy (an event recorded after it has completed), tf_batch_status_unpack
+ * converts `host_raw` into one tf_sim_status per simulation, in the order
+ * the simulations were given to tf_batch_create. */
+size_t tf_batch_status_bytes(const tf_batch *b);
+int tf_batch_status_enqueue(tf_batch *b, void *host_raw, void *stream);
+int tf_batch_status_unpack(const tf_batch *b, const void *host_raw, tf_sim_status *host_out);
 
 /* ------------------------------------------------------------------------
  * Tiling and autotuning of the fused stencil. No reference counterpart.

@@ -146,3 +146,18 @@ def test_server_tick_length_follows_step_cost():
     nearly_done[0].steps_done = 4997
     assert srv._tick_steps([0], nearly_done) == 3  # no more steps than are left
     assert Server(1 << 40, steps_per_tick=7)._tick_steps([0], big) == 7
+
+
+def test_server_next_tick_counts_the_queued_ones():
+    """A group keeps ticks queued ahead of its status reads: the next tick's
+    steps count those queued for each job (a job new to the plan is not held
+    back by ticks it is not in), and a group whose queued ticks cover every
+    job's steps queues no more."""
+    from tinyfluids_amd.engine import Server
+    srv = Server(1 << 40, tick_ms=4.0)
+    jobs = _jobs([_spec((32, 32, 32), 100, 0.0), _spec((32, 32, 32), 40, 0.0)])
+    assert srv._tick_steps([0], jobs, [(None, [0], 60)]) == 40
+    assert srv._tick_steps([0], jobs, [(None, [0], 60), (None, [0], 40)]) == 0
+    # job 1 joined the plan after the queued ticks of job 0 alone
+    assert srv._tick_steps([0, 1], jobs, [(None, [0], 100)]) == 40
+    assert Server(1 << 40, steps_per_tick=7)._tick_steps([0], jobs, [(None, [0], 100)]) == 0

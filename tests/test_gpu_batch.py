@@ -75,6 +75,45 @@ def test_graph_replays_match_eager():
             assert torch.equal(a[i].state(sa[i]), b[i].state(sb[i])), (rep, i)
 
 
+def test_async_plans_and_status_copies_match_synchronous_ones():
+    """Plans built without waiting (tf_batch_create_async), while the stream
+    still runs another plan's steps, and statuses read in two halves
+    (tf_batch_status_enqueue / _unpack, with further steps queued behind the
+    copy) equal the synchronous ones: the simulations are switched between
+    plans of different members mid-run, as a server does."""
+    from tinyfluids_amd.batch import Batch, Simulation
+    specs = specs_mixed()
+    a = [Simulation(s.initial_state(), s.dx, s.gamma, max_steps=s.steps, device=DEV) for s in specs]
+    b = [Simulation(s.initial_state(), s.dx, s.gamma, max_steps=s.steps, device=DEV) for s in specs]
+    for s in a + b:
+        s.reset()
+    ref = Batch(a)
+    stream = torch.cuda.Stream(DEV)  # default priority
+    with torch.cuda.stream(stream):
+        first = Batch(b[:4], asynchronous=True)
+        first.step(3)
+        copy = first.status_async()
+        first.step(2)  # queued behind the copy: the copy sees 3 steps
+        second = Batch(b[2:], asynchronous=True)  # built while `first` may still run
+        second.step(4)
+        copies = [second.status_async()]
+        full = Batch(b, asynchronous=True)
+        full.step(20)
+        copies.append(full.status_async())
+    st3 = copy.result()
+    assert [x.n_iter for x in st3] == [min(3, s.steps) for s in specs[:4]]
+    out = [c.result() for c in copies][-1]
+    stream.synchronize()
+    ref.step(25)
+    want = ref.status()
+    for i, s in enumerate(specs):
+        assert out[i].n_iter == want[i].n_iter == s.steps and not out[i].active
+        assert out[i].t == want[i].t
+        assert torch.equal(b[i].state(out[i]), a[i].state(want[i])), i
+    for plan in (first, second, full, ref):
+        plan.close()
+
+
 def test_whole_run_graph_matches_eager():
     """A whole run as one captured graph (tf_batch_run_graph), cold and
     replayed after the states are restored, equals eager stepping."""

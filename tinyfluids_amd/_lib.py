@@ -78,6 +78,7 @@ SIGNATURES = {
     "tf_sim_reset": (I, [ctypes.POINTER(SimDesc), P]),
     "tf_batch_workspace_bytes": (SZ, [I]),
     "tf_batch_create": (I, [ctypes.POINTER(P), ctypes.POINTER(SimDesc), I, P, SZ, I, I, P]),
+    "tf_batch_create_async": (I, [ctypes.POINTER(P), ctypes.POINTER(SimDesc), I, P, SZ, I, I, P]),
     "tf_batch_destroy": (I, [P]),
     "tf_batch_sim_tiling": (I, [P, I, ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I)]),
     "tf_batch_step": (I, [P, I, P]),
@@ -85,6 +86,9 @@ SIGNATURES = {
     "tf_batch_step_graph": (I, [P, I, P]),
     "tf_batch_run_graph": (I, [P, I, P]),
     "tf_batch_status": (I, [P, ctypes.POINTER(SimStatus), P]),
+    "tf_batch_status_bytes": (SZ, [P]),
+    "tf_batch_status_enqueue": (I, [P, P, P]),
+    "tf_batch_status_unpack": (I, [P, P, ctypes.POINTER(SimStatus)]),
     "tf_batch_step_timed": (I, [P, I, ctypes.POINTER(ctypes.c_float), P]),
     "tf_set_line_mode": (I, [I]),
     "tf_hbm_copy": (I, [P, P, SZ, P]),

@@ -79,8 +79,28 @@ class Simulation:
         return self.buf[int(st.n_iter) & 1]
 
 
+class StatusCopy:
+    """Statuses of a batch being copied to page-locked host memory on the
+    batch's stream (Batch.status_async). Keep it, and its batch open, until
+    result(): the copy writes into its buffer until the stream passes it."""
+
+    def __init__(self, batch: "Batch", raw: torch.Tensor, event):
+        self.batch, self.raw, self.event = batch, raw, event
+
+    def ready(self) -> bool:
+        return self.event.query()
+
+    def result(self) -> List[_lib.SimStatus]:
+        self.event.synchronize()
+        out = (_lib.SimStatus * len(self.batch.sims))()
+        _lib.check(_lib.load().tf_batch_status_unpack(self.batch._h, self.raw.data_ptr(), out))
+        return list(out)
+
+
 class Batch:
-    def __init__(self, sims: Sequence[Simulation], tile_x: int = 0, tile_y: int = 0):
+    def __init__(self, sims: Sequence[Simulation], tile_x: int = 0, tile_y: int = 0, asynchronous: bool = False):
+        """``asynchronous``: build the plan without waiting for the current
+        stream (tf_batch_create_async); step it on that stream."""
         if not sims:
             raise ValueError("empty batch")
         self.sims: List[Simulation] = list(sims)
@@ -90,8 +110,9 @@ class Batch:
         self.workspace = torch.empty(lib.tf_batch_workspace_bytes(n), dtype=torch.uint8, device=device)
         descs = (_lib.SimDesc * n)(*[s.desc for s in self.sims])
         handle = ctypes.c_void_p()
-        _lib.check(lib.tf_batch_create(ctypes.byref(handle), descs, n, self.workspace.data_ptr(),
-                                       self.workspace.numel(), int(tile_x), int(tile_y), _stream(device)))
+        create = lib.tf_batch_create_async if asynchronous else lib.tf_batch_create
+        _lib.check(create(ctypes.byref(handle), descs, n, self.workspace.data_ptr(), self.workspace.numel(),
+                          int(tile_x), int(tile_y), _stream(device)))
         self._h = handle
         self.device = device
 
@@ -132,6 +153,16 @@ class Batch:
         ms = ctypes.c_float(0.0)
         _lib.check(_lib.load().tf_batch_step_timed(self._h, int(steps), ctypes.byref(ms), _stream(self.device)))
         return float(ms.value)
+
+    def status_async(self) -> StatusCopy:
+        """Enqueues a copy of every simulation's status on the current stream
+        and returns at once; StatusCopy.result() waits for it."""
+        lib = _lib.load()
+        raw = torch.empty(lib.tf_batch_status_bytes(self._h), dtype=torch.uint8, pin_memory=True)
+        _lib.check(lib.tf_batch_status_enqueue(self._h, raw.data_ptr(), _stream(self.device)))
+        event = torch.cuda.Event()
+        event.record(torch.cuda.current_stream(self.device))
+        return StatusCopy(self, raw, event)
 
     def status(self) -> List[_lib.SimStatus]:
         n = len(self.sims)

@@ -598,8 +598,83 @@ int tf_sim_reset(const tf_sim_desc *sim, void *stream) {
     return TF_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// Page-locked staging blocks for plan-table uploads that do not wait
+// (tf_batch_create_async). A block is reused once the event recorded after
+// its last upload has completed; blocks are kept until the process ends, so
+// no page-locked free (which may synchronise the device) is ever needed.
+struct Staging {
+    unsigned char *host = nullptr;
+    size_t bytes = 0;
+    int device = -1;
+    bool in_use = false;
+    hipEvent_t done = nullptr;
+};
+std::mutex g_staging_mu;
+std::vector<Staging> g_staging;
+
+// Index of a free block of at least `bytes` on `device`, allocated if none is.
+int staging_acquire(size_t bytes, int device, hipError_t &err) {
+    std::lock_guard<std::mutex> lk(g_staging_mu);
+    for (size_t i = 0; i < g_staging.size(); ++i) {
+        Staging &s = g_staging[i];
+        if (s.in_use || s.device != device || s.bytes < bytes) continue;
+        if (s.done && hipEventQuery(s.done) != hipSuccess) continue;
+        s.in_use = true;
+        return (int)i;
+    }
+    Staging s;
+    s.bytes = std::max<size_t>(bytes, 64 << 10);
+    s.device = device;
+    err = hipHostMalloc((void **)&s.host, s.bytes, hipHostMallocDefault);
+    if (err != hipSuccess) return -1;
+    err = hipEventCreateWithFlags(&s.done, hipEventDisableTiming);
+    if (err != hipSuccess) {
+        (void)hipHostFree(s.host);  // error path only
+        return -1;
+    }
+    s.in_use = true;
+    g_staging.push_back(s);
+    return (int)g_staging.size() - 1;
+}
+
+// Records the block's event on `st` after its uploads and releases it.
+hipError_t staging_release(int i, hipStream_t st) {
+    std::lock_guard<std::mutex> lk(g_staging_mu);
+    Staging &s = g_staging[i];
+    const hipError_t e = hipEventRecord(s.done, st);
+    s.in_use = false;
+    return e;
+}
+
+unsigned char *staging_host(int i) {
+    std::lock_guard<std::mutex> lk(g_staging_mu);
+    return g_staging[i].host;
+}
+
+int batch_create(tf_batch **out, const tf_sim_desc *sims, int nsims, void *workspace, size_t workspace_bytes,
+                 int tile_x, int tile_y, void *stream, bool wait);
+}  // namespace
+
+extern "C" {
+
 int tf_batch_create(tf_batch **out, const tf_sim_desc *sims, int nsims, void *workspace, size_t workspace_bytes,
                     int tile_x, int tile_y, void *stream) {
+    return batch_create(out, sims, nsims, workspace, workspace_bytes, tile_x, tile_y, stream, true);
+}
+
+int tf_batch_create_async(tf_batch **out, const tf_sim_desc *sims, int nsims, void *workspace,
+                          size_t workspace_bytes, int tile_x, int tile_y, void *stream) {
+    return batch_create(out, sims, nsims, workspace, workspace_bytes, tile_x, tile_y, stream, false);
+}
+
+}  // extern "C"
+
+namespace {
+int batch_create(tf_batch **out, const tf_sim_desc *sims, int nsims, void *workspace, size_t workspace_bytes,
+                 int tile_x, int tile_y, void *stream, bool wait) {
     TF_REQUIRE(out, "null output handle");
     *out = nullptr;
     TF_REQUIRE(nsims >= 1, "nsims must be >= 1");
@@ -677,12 +752,34 @@ int tf_batch_create(tf_batch **out, const tf_sim_desc *sims, int nsims, void *wo
     b->nlaunch = P.nlaunch;
     // The tables in one upload of their host image, the far tiles (when there
     // are any) in a second; the signal-speed slots are cleared on the device.
+    // Waiting: from the host vectors, then the stream is synchronised. Not
+    // waiting: from a page-locked staging block, guarded by an event.
     hipStream_t st = S_(stream);
-    hipError_t e = hipMemsetAsync(base + P.off_maxp, 0, (size_t)nsims * kSlotBytesPerSim, st);  // not in a graph
-    if (e == hipSuccess) e = hipMemcpyAsync(base, P.image.data(), P.image.size(), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess && b->nfar > 0)
-        e = hipMemcpyAsync(b->d_far, P.far.data(), P.far.size() * sizeof(FarTile), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    const size_t far_bytes = P.far.size() * sizeof(FarTile);
+    const void *image = P.image.data();
+    const void *far = P.far.data();
+    int slot = -1;
+    hipError_t e = hipSuccess;
+    if (!wait) {
+        int dev = 0;
+        e = hipGetDevice(&dev);
+        if (e == hipSuccess) slot = staging_acquire(P.image.size() + far_bytes, dev, e);
+        if (slot >= 0) {
+            unsigned char *h = staging_host(slot);
+            std::memcpy(h, image, P.image.size());
+            if (far_bytes) std::memcpy(h + P.image.size(), far, far_bytes);
+            image = h;
+            far = h + P.image.size();
+        }
+    }
+    if (e == hipSuccess) e = hipMemsetAsync(base + P.off_maxp, 0, (size_t)nsims * kSlotBytesPerSim, st);  // not in a graph
+    if (e == hipSuccess) e = hipMemcpyAsync(base, image, P.image.size(), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && b->nfar > 0) e = hipMemcpyAsync(b->d_far, far, far_bytes, hipMemcpyHostToDevice, st);
+    if (slot >= 0) {
+        const hipError_t r = staging_release(slot, st);
+        if (e == hipSuccess) e = r;
+    }
+    if (e == hipSuccess && wait) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
         delete b;
         return fail(TF_ERR_HIP, "batch table upload: %s", hipGetErrorString(e));
@@ -690,6 +787,9 @@ int tf_batch_create(tf_batch **out, const tf_sim_desc *sims, int nsims, void *wo
     *out = b;
     return TF_OK;
 }
+}  // namespace
+
+extern "C" {
 
 int tf_batch_sim_tiling(const tf_batch *b, int sim, int *tile_x, int *tile_y, int *line_waves) {
     TF_REQUIRE(b && tile_x && tile_y && line_waves, "null pointer");
@@ -892,15 +992,31 @@ static_assert(sizeof(tf_sim_status) == 48 && offsetof(Ctrl, t) == offsetof(tf_si
                   offsetof(Ctrl, max_speed) == offsetof(tf_sim_status, max_speed),
               "Ctrl must begin with the layout of tf_sim_status");
 
-int tf_batch_status(tf_batch *b, tf_sim_status *host_out, void *stream) {
-    TF_REQUIRE(b && host_out, "null pointer");
+size_t tf_batch_status_bytes(const tf_batch *b) { return b ? (size_t)b->nsims * sizeof(Ctrl) : 0; }
+
+// The control blocks in plan order are gathered into the plan's status
+// buffer and copied to the caller's host memory, both on `stream`.
+int tf_batch_status_enqueue(tf_batch *b, void *host_raw, void *stream) {
+    TF_REQUIRE(b && host_raw, "null pointer");
     hipStream_t st = S_(stream);
     TF_HIP(launch_gather_status(b->d_sims, b->nsims, b->d_status, st));
-    std::vector<Ctrl> tmp(b->nsims);
-    TF_HIP(hipMemcpyAsync(tmp.data(), b->d_status, b->nsims * sizeof(Ctrl), hipMemcpyDeviceToHost, st));
-    TF_HIP(hipStreamSynchronize(st));
-    for (int k = 0; k < b->nsims; ++k) std::memcpy(&host_out[b->order[k]], &tmp[k], sizeof(tf_sim_status));
+    TF_HIP(hipMemcpyAsync(host_raw, b->d_status, b->nsims * sizeof(Ctrl), hipMemcpyDeviceToHost, st));
     return TF_OK;
+}
+
+int tf_batch_status_unpack(const tf_batch *b, const void *host_raw, tf_sim_status *host_out) {
+    TF_REQUIRE(b && host_raw && host_out, "null pointer");
+    const Ctrl *c = static_cast<const Ctrl *>(host_raw);
+    for (int k = 0; k < b->nsims; ++k) std::memcpy(&host_out[b->order[k]], &c[k], sizeof(tf_sim_status));
+    return TF_OK;
+}
+
+int tf_batch_status(tf_batch *b, tf_sim_status *host_out, void *stream) {
+    TF_REQUIRE(b && host_out, "null pointer");
+    std::vector<Ctrl> tmp(b->nsims);
+    if (int e = tf_batch_status_enqueue(b, tmp.data(), stream)) return e;
+    TF_HIP(hipStreamSynchronize(S_(stream)));
+    return tf_batch_status_unpack(b, tmp.data(), host_out);
 }
 
 }  // extern "C"

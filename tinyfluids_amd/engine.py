@@ -25,6 +25,7 @@ from __future__ import annotations
 
 import math
 import time
+from collections import deque
 from concurrent.futures import FIRST_COMPLETED, ThreadPoolExecutor, wait
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Sequence
@@ -210,7 +211,7 @@ class Server:
 
     def __init__(self, hbm_budget_bytes: int, max_resident: int = 1024, steps_per_tick: int = 0, device=None,
                  keep_states: bool = False, tile_x: int = 0, fill_cells: int = 1 << 24, tick_ms: float = 4.0,
-                 policy: str = "shortest", prep_threads: int = 4, upload_threads: int = 2):
+                 policy: str = "shortest", prep_threads: int = 4, upload_threads: int = 2, depth: int = 2):
         if policy not in ("shortest", "fcfs"):
             raise ValueError(f"policy must be 'shortest' or 'fcfs', got {policy!r}")
         self.policy = policy
@@ -224,6 +225,7 @@ class Server:
         self.tick_ms = float(tick_ms)
         self.prep_threads = max(1, int(prep_threads))
         self.upload_threads = max(1, int(upload_threads))
+        self.depth = max(1, int(depth))
 
     @classmethod
     def group_of(cls, spec: SimSpec) -> int:
@@ -244,12 +246,20 @@ class Server:
             cells += jobs[i].spec.cells
         return sorted(chosen)
 
-    def _tick_steps(self, ids: List[int], jobs: List[JobRecord]) -> int:
+    def _tick_steps(self, ids: List[int], jobs: List[JobRecord], queued=()) -> int:
+        """Steps of a group's next tick; 0 when the ticks already queued
+        (``queued``: (plan, ids, steps, ...) tuples) cover every job's steps."""
+        ahead: Dict[int, int] = {}
+        for t in queued:
+            for i in t[1]:
+                ahead[i] = ahead.get(i, 0) + t[2]
+        left = max(jobs[i].spec.steps - jobs[i].steps_done - ahead.get(i, 0) for i in ids)
+        if left <= 0:
+            return 0
         if self.steps_per_tick > 0:
             return self.steps_per_tick
         cells = sum(jobs[i].spec.cells for i in ids)
         step_us = 12.0 + cells * 40.0 / 4.5e6  # launch overhead + HBM traffic at ~4.5 TB/s
-        left = max(jobs[i].spec.steps - jobs[i].steps_done for i in ids)
         return int(max(1, min(left, 1024, self.tick_ms * 1e3 / step_us)))
 
     def serve(self, specs: Sequence[SimSpec], time_scale: float = 1.0,
@@ -299,23 +309,35 @@ class Server:
         uploading = {}  # job index -> future of (Simulation, event) on the upload pool
         pending = list(jobs)  # in arrival order
         queue: List[JobRecord] = []
-        active: Dict[int, Simulation] = {}
-        batches: List[Optional[Batch]] = [None] * ngroups
+        active: Dict[int, Simulation] = {}  # resident jobs, until their memory is released
+        finished = set()  # finished jobs whose buffers a queued tick still holds
+        holds: Dict[int, int] = {}  # job index -> queued ticks whose plan includes it
+        batches: List[Optional[Batch]] = [None] * ngroups  # each group's current plan
         ids: List[List[int]] = [[] for _ in range(ngroups)]
+        queued: List[deque] = [deque() for _ in range(ngroups)]  # (plan, ids, steps, StatusCopy)
         used = peak = ticks = max_res = done = 0
         t_wait = t_plan = 0.0  # host seconds waiting for tick outcomes / rebuilding plans
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         while done < len(jobs):
-            # 1. enqueue the next tick of every group (asynchronous, one stream each)
-            ticked = [b is not None for b in batches]
-            for g, b in enumerate(batches):
-                if b is not None:
+            # 1. every group keeps `depth` ticks queued on its current plan, each
+            #    followed by an asynchronous copy of its statuses
+            for g in range(ngroups):
+                b = batches[g]
+                while b is not None and len(queued[g]) < self.depth:
+                    n = self._tick_steps(ids[g], jobs, queued[g])
+                    if n <= 0:
+                        break
                     with torch.cuda.stream(streams[g]):
-                        b.step(self._tick_steps(ids[g], jobs))
-            ticks += any(ticked)
-            # 2. admission in arrival order while memory and slots allow; the
-            #    host prepares new jobs while the device runs the tick
+                        b.step(n)
+                        queued[g].append((b, ids[g], n, b.status_async()))
+                    for i in ids[g]:
+                        holds[i] = holds.get(i, 0) + 1
+                    ticks += 1
+            changed = False
+            # 2. arrivals; admission in arrival order while memory and slots
+            #    allow (memory and a slot are reserved now, the upload runs on
+            #    the upload pool); uploaded jobs join their group
             now = time.perf_counter() - t0
             while pending and pending[0].arrival <= now:
                 j = pending.pop(0)
@@ -324,15 +346,12 @@ class Server:
                     prepared[j.index] = pool.submit(lambda a=self._initial[j.index]: a)
                 else:
                     prepared[j.index] = pool.submit(j.spec.initial_state)
-            # admission in arrival order: memory and a slot are reserved now,
-            # the upload runs on the upload pool
             while (queue and len(active) + len(uploading) < self.max_resident
                    and used + state_bytes(queue[0].spec) <= self.budget):
                 j = queue.pop(0)
                 uploading[j.index] = uploader.submit(self._upload, j.spec, prepared.pop(j.index), upload)
                 used += state_bytes(j.spec)
                 j.admitted = time.perf_counter() - t0
-            # uploaded jobs join their group
             for i in [i for i, f in uploading.items() if f.done()]:
                 sim, ev = uploading.pop(i).result()
                 g = self.group_of(jobs[i].spec)
@@ -346,57 +365,75 @@ class Server:
                 with torch.cuda.stream(streams[g]):
                     sim.reset()
                 active[i] = sim
+                changed = True
             peak = max(peak, used)
             max_res = max(max_res, len(active))
-            # 3. each group's tick outcome: finished jobs leave and free their memory
-            for g, b in enumerate(batches):
-                if not ticked[g]:
-                    continue
-                with torch.cuda.stream(streams[g]):
+            # 3. completed ticks, oldest first in each group: finished jobs
+            #    leave their group's plan, and their memory is released once no
+            #    queued tick holds it; a replaced plan is closed after its last
+            #    tick
+            for g in range(ngroups):
+                while queued[g] and queued[g][0][3].ready():
+                    b, tids, _, copy = queued[g].popleft()
                     tw = time.perf_counter()
-                    statuses = b.status()  # waits for this group's tick
+                    statuses = copy.result()
                     t_wait += time.perf_counter() - tw
+                    changed = True
                     now = time.perf_counter() - t0
-                    for i, st in zip(ids[g], statuses):
-                        jobs[i].steps_done = int(st.n_iter)
-                        if not st.active:
-                            j = jobs[i]
-                            sim = active.pop(i)
+                    for i, st in zip(tids, statuses):
+                        holds[i] -= 1
+                        j = jobs[i]
+                        j.steps_done = max(j.steps_done, int(st.n_iter))
+                        if not st.active and i not in finished and math.isnan(j.finished):
                             j.finished = now
                             j.t = float(st.t)
                             j.nonfinite = bool(st.nonfinite)
                             if self.keep_states:
-                                j.state = sim.state(st).clone()
-                            used -= state_bytes(j.spec)
+                                with torch.cuda.stream(streams[g]):
+                                    j.state = active[i].state(st).clone()
+                            finished.add(i)
                             done += 1
-            # 4. a group's plan is rebuilt when its chosen jobs change
-            tp = time.perf_counter()
-            members = [[] for _ in range(ngroups)]
-            for i in active:
-                members[self.group_of(jobs[i].spec)].append(i)
-            for g in range(ngroups):
-                chosen = self._select(members[g], jobs) if members[g] else []
-                if chosen != ids[g] or (batches[g] is None and chosen):
+                    for i in [i for i in finished if holds[i] == 0]:
+                        finished.discard(i)
+                        del active[i]
+                        used -= state_bytes(jobs[i].spec)
+                    if b is not batches[g] and all(t[0] is not b for t in queued[g]):
+                        b.close()
+            # 4. a group's plan is rebuilt when its chosen jobs change (built
+            #    without waiting: the group's queued ticks keep running)
+            if changed:
+                tp = time.perf_counter()
+                members = [[] for _ in range(ngroups)]
+                for i in active:
+                    if i not in finished:
+                        members[self.group_of(jobs[i].spec)].append(i)
+                for g in range(ngroups):
+                    chosen = self._select(members[g], jobs) if members[g] else []
+                    if chosen == ids[g] and (batches[g] is not None or not chosen):
+                        continue
+                    old = batches[g]
+                    if old is not None and all(t[0] is not old for t in queued[g]):
+                        old.close()
+                    ids[g] = chosen
                     with torch.cuda.stream(streams[g]):
-                        if batches[g] is not None:
-                            batches[g].close()
-                            batches[g] = None
-                        ids[g] = chosen
-                        if chosen:
-                            batches[g] = Batch([active[i] for i in chosen], tile_x=self.tile_x)
-            t_plan += time.perf_counter() - tp
-            if all(b is None for b in batches):
-                if uploading:
-                    # idle: wait for an upload
-                    wait(list(uploading.values()), timeout=0.01, return_when=FIRST_COMPLETED)
-                elif pending and not queue:
-                    # idle: wait for the next arrival
-                    time.sleep(max(0.0, min(0.01, pending[0].arrival - (time.perf_counter() - t0))))
-        for g, b in enumerate(batches):
-            if b is not None:
-                with torch.cuda.stream(streams[g]):
-                    b.close()
+                        batches[g] = Batch([active[i] for i in chosen], tile_x=self.tile_x,
+                                           asynchronous=True) if chosen else None
+                t_plan += time.perf_counter() - tp
+            elif any(queued):
+                time.sleep(2e-4)  # the device is busy: poll its ticks
+            elif uploading:
+                wait(list(uploading.values()), timeout=0.01, return_when=FIRST_COMPLETED)
+            elif pending and not queue:
+                # idle: wait for the next arrival
+                time.sleep(max(0.0, min(0.01, pending[0].arrival - (time.perf_counter() - t0))))
         torch.cuda.synchronize()
+        for g in range(ngroups):
+            plans = {id(t[0]): t[0] for t in queued[g]}
+            if batches[g] is not None:
+                plans[id(batches[g])] = batches[g]
+            queued[g].clear()
+            for b in plans.values():
+                b.close()
         makespan = time.perf_counter() - t0
         return ServeReport(jobs, makespan, ticks, max_res, peak,
                            extra={"status_wait_s": t_wait, "plan_rebuild_s": t_plan})
