@@ -640,12 +640,14 @@ int staging_acquire(size_t bytes, int device, hipError_t &err) {
     return (int)g_staging.size() - 1;
 }
 
-// Records the block's event on `st` after its uploads and releases it.
+// Records the block's event on `st` after its uploads and releases it. If the
+// record fails, the block stays in use for good: an event that was never
+// recorded would report its uploads complete.
 hipError_t staging_release(int i, hipStream_t st) {
     std::lock_guard<std::mutex> lk(g_staging_mu);
     Staging &s = g_staging[i];
     const hipError_t e = hipEventRecord(s.done, st);
-    s.in_use = false;
+    if (e == hipSuccess) s.in_use = false;
     return e;
 }
 
