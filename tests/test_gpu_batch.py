@@ -192,6 +192,37 @@ def test_server_results_same_alone_and_in_mix():
         assert abs(a.t - t_ref) < 1e-6 * t_ref, (a.index, a.t, t_ref)
 
 
+@pytest.mark.parametrize("steps_per_tick,depth", [(1, 3), (2, 2), (0, 1)])
+def test_server_pipeline_churn_bitwise(steps_per_tick, depth):
+    """The pipelined server under churn: ticks of one or two steps with two
+    or three ticks queued per group, so plans are rebuilt while the old
+    plans' ticks are still queued, finished jobs are released a tick later,
+    and kept states are cloned behind queued ticks; also one tick queued at a
+    time. Jobs of all three size groups arrive in bursts. Every result
+    equals the job run alone, bit for bit, with the same step count and t,
+    and the HBM budget holds."""
+    from tinyfluids_amd import engine
+    specs = specs_mixed() + [
+        ics.SimSpec((128, 96, 96), ics.GAMMA_KH, 1 / 96, 5, "kh"),
+        ics.SimSpec((256, 256, 128), ics.GAMMA_BLAST, 1 / 256, 4, "blast", (0.5, 0.4, 0.5), 0.15, 20.0),
+        ics.SimSpec((20, 24, 28), ics.GAMMA_BLAST, 1 / 28, 13, "blast", (0.5, 0.5, 0.5), 0.25, 8.0),
+        ics.SimSpec((64, 64, 64), ics.GAMMA_KH, 1 / 64, 3, "kh"),
+    ]
+    for k, s in enumerate(specs):
+        s.arrival = 0.001 * (k // 3)  # bursts of three
+    budget = 1 << 30
+    srv = engine.Server(hbm_budget_bytes=budget, steps_per_tick=steps_per_tick, depth=depth, keep_states=True,
+                        tick_ms=0.05)
+    rep = srv.serve(specs)
+    alone = engine.serve_plain(specs, keep_states=True)
+    assert rep.peak_bytes <= budget
+    assert {engine.Server.group_of(s) for s in specs} == {0, 1, 2}
+    for a, b in zip(rep.jobs, alone.jobs):
+        assert a.steps_done == a.spec.steps and not a.nonfinite, a.index
+        assert a.t == b.t, a.index
+        assert torch.equal(a.state, b.state), a.index
+
+
 def test_server_respects_hbm_budget():
     from tinyfluids_amd import engine
     specs = [ics.SimSpec((32, 32, 32), ics.GAMMA_KH, 1 / 32, 6, "kh", arrival=0.0) for _ in range(5)]
