@@ -301,17 +301,18 @@ def test_line_tiling_bitwise(line_mode, tile_x):
 
 
 def test_tune_batch_picks_a_measured_candidate():
-    """The batch autotuner times every candidate tile width on the real batch
-    and returns the fastest; the run with it matches the default plan."""
+    """The batch autotuner times every candidate tile width and y-chunk (the
+    plan's default chunks and 8, 16, 32 rows) on the real batch and returns
+    the fastest; the run with it matches the default plan."""
     from tinyfluids_amd import engine
     specs = specs_mixed()
     init = engine.upload(specs, DEV)
     tuned = engine.tune_batch(specs, init, candidates=(2, 4), steps=2)
-    assert tuned["tile_x"] in (2, 4)
-    assert set(tuned["candidates"]) == {"2,0", "4,0"}
+    assert tuned["tile_x"] in (2, 4) and tuned["tile_y"] in (0, 8, 16, 32)
+    assert set(tuned["candidates"]) == {f"{tx},{ty}" for tx in (2, 4) for ty in (0, 8, 16, 32)}
     assert all(v > 0 and math.isfinite(v) for v in tuned["candidates"].values())
     assert tuned["ms_per_step"] == min(tuned["candidates"].values())
     ref = engine.run_batched(specs, init, graph=False)
-    b = engine.run_batched(specs, init, graph=True, tile_x=tuned["tile_x"])
+    b = engine.run_batched(specs, init, graph=True, tile_x=tuned["tile_x"], tile_y=tuned["tile_y"])
     for x, y in zip(b.states, ref.states):
         assert torch.equal(x, y)
