@@ -49,6 +49,17 @@ def test_server_rejects_jobs_larger_than_budget():
         srv.serve([big])
 
 
+def test_serving_rejects_a_wrong_number_of_initial_states():
+    torch = pytest.importorskip("torch")
+    from tinyfluids_amd import engine
+    specs = [ics.SimSpec((8, 8, 8), 1.4, 1 / 8, 5, "kh") for _ in range(3)]
+    initial = [s.initial_state() for s in specs[:2]]
+    with pytest.raises(ValueError, match="2 initial states for 3 jobs"):
+        engine.Server(hbm_budget_bytes=1 << 30, device=torch.device("cpu")).serve(specs, initial=initial)
+    with pytest.raises(ValueError, match="2 initial states for 3 jobs"):
+        engine.serve_plain(specs, device=torch.device("cpu"), initial=initial)
+
+
 def test_setup_ics_matches_reference_harness():
     s, dx = ics.setup_ics(32, 2)
     assert dx == 1 / 31

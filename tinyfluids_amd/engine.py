@@ -282,7 +282,8 @@ class Server:
         a host-to-device copy, so the ticks it has queued keep the device
         busy, and a group stream waits only for its own job's upload."""
         jobs = [JobRecord(i, s, s.arrival * time_scale) for i, s in enumerate(specs)]
-        self._initial = initial
+        if initial is not None and len(initial) != len(jobs):
+            raise ValueError(f"{len(initial)} initial states for {len(jobs)} jobs")
         for j in jobs:
             if state_bytes(j.spec) > self.budget:
                 raise ValueError(f"job {j.index} ({j.spec.shape}) does not fit the HBM budget")
@@ -294,7 +295,7 @@ class Server:
         # must not occupy the threads that build those states
         with ThreadPoolExecutor(max_workers=self.prep_threads) as pool, \
                 ThreadPoolExecutor(max_workers=self.upload_threads) as uploader:
-            return self._serve(jobs, streams, upload, pool, uploader)
+            return self._serve(jobs, initial, streams, upload, pool, uploader)
 
     def _upload(self, spec: SimSpec, host_state, upload) -> tuple:
         """Runs on an upload thread: the job's buffers and control block,
@@ -308,7 +309,7 @@ class Server:
             ev.record(upload)
         return sim, ev
 
-    def _serve(self, jobs: List[JobRecord], streams, upload, pool, uploader) -> ServeReport:
+    def _serve(self, jobs: List[JobRecord], initial, streams, upload, pool, uploader) -> ServeReport:
         ngroups = len(streams)
         prepared = {}  # job index -> future of its initial state (host array)
         uploading = {}  # job index -> future of (Simulation, event) on the upload pool
@@ -347,8 +348,8 @@ class Server:
             while pending and pending[0].arrival <= now:
                 j = pending.pop(0)
                 queue.append(j)
-                if self._initial is not None:
-                    prepared[j.index] = pool.submit(lambda a=self._initial[j.index]: a)
+                if initial is not None:
+                    prepared[j.index] = pool.submit(lambda a=initial[j.index]: a)
                 else:
                     prepared[j.index] = pool.submit(j.spec.initial_state)
             while (queue and len(active) + len(uploading) < self.max_resident
@@ -450,6 +451,8 @@ def serve_plain(specs: Sequence[SimSpec], time_scale: float = 1.0, device=None,
     tilings of the stream's grid shapes measured beforehand). ``initial`` as
     for Server.serve; without it each job's state is built when it starts."""
     device = device or torch.device("cuda:0")
+    if initial is not None and len(initial) != len(specs):
+        raise ValueError(f"{len(initial)} initial states for {len(specs)} jobs")
     tune_plain(specs, device)
     jobs = [JobRecord(i, s, s.arrival * time_scale) for i, s in enumerate(specs)]
     torch.cuda.synchronize()
