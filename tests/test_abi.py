@@ -90,6 +90,25 @@ def test_argument_errors_without_gpu():
         _lib.check(_lib.TF_ERR_ARG)
 
 
+def test_batch_entry_points_check_arguments_without_gpu():
+    """The batch plan and status entry points (synchronous and asynchronous
+    forms) answer argument errors before any device work."""
+    from tinyfluids_amd import _lib
+    lib = _lib.load()
+    handle = ctypes.c_void_p()
+    for create in (lib.tf_batch_create, lib.tf_batch_create_async):
+        assert create(ctypes.byref(handle), None, 0, ctypes.c_void_p(16), 1 << 20, 0, 0, None) == _lib.TF_ERR_ARG
+        assert b"nsims" in lib.tf_last_error()
+        assert create(None, None, 1, None, 0, 0, 0, None) == _lib.TF_ERR_ARG
+        assert b"null" in lib.tf_last_error()
+    assert lib.tf_batch_status_bytes(None) == 0
+    assert lib.tf_batch_status_enqueue(None, ctypes.c_void_p(16), None) == _lib.TF_ERR_ARG
+    assert b"null" in lib.tf_last_error()
+    out = (_lib.SimStatus * 1)()
+    assert lib.tf_batch_status_unpack(None, ctypes.c_void_p(16), out) == _lib.TF_ERR_ARG
+    assert lib.tf_batch_status(None, out, None) == _lib.TF_ERR_ARG
+
+
 def test_python_front_end_rejects_bad_states():
     import torch
     import tinyfluids_amd as tf
