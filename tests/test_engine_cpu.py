@@ -1,7 +1,6 @@
 """Host-side logic of the workloads (no GPU): workload generators and the
 server's admission checks."""
 
-import math
 
 import numpy as np
 import pytest

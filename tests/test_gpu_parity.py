@@ -7,7 +7,6 @@ fields that are zero in the oracle):
   - full runs (100-200 steps through shocks): 1e-5
 """
 
-import math
 
 import numpy as np
 import pytest

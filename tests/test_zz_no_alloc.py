@@ -7,7 +7,6 @@ launch."""
 
 import ctypes
 
-import numpy as np
 import pytest
 import torch
 
